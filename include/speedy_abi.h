@@ -1,0 +1,173 @@
+/*
+ * speedy_abi.h -- C ABI of the MI355X (gfx950) SPEEDY column-physics library.
+ *
+ * The library replaces, for the SPEEDY physics suite, the call
+ *     SpeedyPhysics.compute_tendencies(state, forcing, geometry, date)
+ *       reference: jcm/physics/speedy/speedy_physics.py:88-127
+ *       plugin API: jcm/physics_interface.py:137-155 (Physics.compute_tendencies)
+ * i.e. the eleven physics terms listed at speedy_physics.py:67-80, summed in
+ * term order.  Plain pointers and sizes only: every array pointer is a device
+ * pointer (hipMalloc'ed or from a framework), every struct is host memory
+ * that is read during the call and may be freed afterwards.
+ *
+ * Layout (structure of arrays, level-major): a 3-D field is float[kx][ncol],
+ * a 2-D field float[ncol], with ncol = nlon*nlat and column c = i*nlat + j
+ * (latitude fastest, south to north).  This is the C-order layout of the
+ * reference's (kx, ix, il) arrays (jcm/geometry.py:89,130), so framework
+ * buffers are consumed without copies.  Members of an ensemble are stacked
+ * on a leading axis: state/tendencies float[nmember][kx][ncol],
+ * ps float[nmember][ncol]; forcing and geometry are shared by all members.
+ *
+ * Errors: every entry point returns SPEEDY_OK (0) or a nonzero code and never
+ * aborts; speedy_last_error() gives a message for the calling thread.  NaN
+ * inputs propagate as in the reference except through min/max, where the
+ * GPU's IEEE maxNum/minNum return the non-NaN operand.
+ */
+#ifndef SPEEDY_ABI_H
+#define SPEEDY_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPEEDY_ABI_VERSION 1
+#define SPEEDY_KX_MAX 16
+
+enum {
+  SPEEDY_OK = 0,
+  SPEEDY_ERR_ARG = 1,        /* null pointer, bad size, bad flag */
+  SPEEDY_ERR_KX = 2,         /* kx not in {7, 8, 16} */
+  SPEEDY_ERR_WORKSPACE = 3,  /* workspace missing or too small */
+  SPEEDY_ERR_HIP = 4,        /* a HIP runtime call failed */
+  SPEEDY_ERR_CAPACITY = 5    /* request exceeds a stated maximum */
+};
+
+/* flags */
+#define SPEEDY_FLAG_LFLUXLAND 0x1u   /* ForcingData.lfluxland (forcing.py:17) */
+
+/* term_mask bit i selects the tendency of term i (speedy_physics.py:67-80)
+ * for the sum; every term always runs, so the data handed between terms are
+ * unchanged.  SPEEDY_TERMS_ALL is the reference's composition. */
+enum {
+  SPEEDY_TERM_FLAGS = 0, SPEEDY_TERM_FORCING, SPEEDY_TERM_HUMIDITY, SPEEDY_TERM_CONVECTION,
+  SPEEDY_TERM_CONDENSATION, SPEEDY_TERM_CLOUDS, SPEEDY_TERM_SHORTWAVE, SPEEDY_TERM_LW_DOWN,
+  SPEEDY_TERM_SURFACE, SPEEDY_TERM_LW_UP, SPEEDY_TERM_VDIFF, SPEEDY_NTERMS
+};
+#define SPEEDY_TERMS_ALL ((1u << SPEEDY_NTERMS) - 1u)
+
+/* Parameters.default() (jcm/physics/speedy/params.py:244-254), packed in
+ * declaration order, plus two multipliers of this library (co2_scale on
+ * ablco2, drag_scale on cdl and cds; 1.0 reproduces the reference). */
+typedef struct {
+  float psmin, trcnv, rhil, rhbl, entmax, smf;                       /* params.py:11-16 */
+  float trlsc, rhlsc, drhlsc, rhblsc;                                /* params.py:51-54 */
+  float albcl, albcls, absdry, absaer, abswv1, abswv2, abscl1, abscl2,
+        ablwin, ablwv1, ablwv2, ablcl1, ablcl2, rhcl1, rhcl2, qacl, wpcl,
+        pmaxcl, clsmax, clsminl, gse_s0, gse_s1;                     /* params.py:72-99 */
+  float albsea, albice, albsn, epslw, emisfc;                        /* params.py:134-140 */
+  float fwind0, ftemp0, fhum0, cdl, cds, chl, chs, vgust, ctday, dtheta,
+        fstab, clambda, clambsn, hdrag;                              /* params.py:157-183 */
+  float trshc, trvdi, trvds, redshc, rhgrad, segrad;                 /* params.py:213-218 */
+  float co2_scale, drag_scale;                                       /* library multipliers */
+  int32_t lscasym, lskineb, increase_co2, co2_year_ref;              /* params.py:180-181,34-35 */
+} SpeedyParams;
+
+/* Per-level constants of Geometry (jcm/geometry.py:62-85). */
+typedef struct {
+  int32_t kx;
+  float hsg[SPEEDY_KX_MAX + 1];
+  float fsg[SPEEDY_KX_MAX], dhs[SPEEDY_KX_MAX], sigl[SPEEDY_KX_MAX];
+  float grdsig[SPEEDY_KX_MAX], grdscp[SPEEDY_KX_MAX];
+  float wvi[SPEEDY_KX_MAX][2];
+} SpeedyVertical;
+
+/* DateData (jcm/date.py:9-48). */
+typedef struct {
+  float tyear;
+  int32_t model_year;
+  int32_t model_step;
+  float dt_seconds;
+} SpeedyDate;
+
+/* Device pointers of one call.  Inputs are read-only. */
+typedef struct {
+  /* PhysicsState (physics_interface.py:22-29): [nmember][kx][ncol]; ps [nmember][ncol] */
+  const float *u, *v, *t, *q, *phi, *ps;
+  /* ForcingData after the day slice (forcing.py:8-17): [ncol] */
+  const float *alb0, *sice_am, *snowc_am, *soilw_am, *stl_am, *sst;
+  /* Geometry (geometry.py:91-97): fmask, orog, phis0 [ncol]; sia, coa [nlat] */
+  const float *fmask, *orog, *phis0, *sia, *coa;
+  /* PhysicsTendency (physics_interface.py:89-94): [nmember][kx][ncol] */
+  float *du, *dv, *dt, *dq;
+  /* optional PhysicsData subset, [nmember][speedy_diag_rows(kx)][ncol], or NULL */
+  float *diag;
+} SpeedyFields;
+
+/* Cotangents and gradients of the fused forward+VJP call. */
+typedef struct {
+  /* cotangents of the tendencies, [nmember][kx][ncol] */
+  const float *ct_du, *ct_dv, *ct_dt, *ct_dq;
+  /* gradients w.r.t. the state, [nmember][kx][ncol]; g_ps [nmember][ncol] */
+  float *g_u, *g_v, *g_t, *g_q, *g_ps;
+  /* gradients w.r.t. (rhbl, trcnv, rhlsc, co2_scale, drag_scale) summed over
+   * the columns of each member: float[nmember][5] */
+  float *g_params;
+} SpeedyAdjoint;
+
+/* Version of this ABI (SPEEDY_ABI_VERSION). */
+int speedy_abi_version(void);
+
+/* Bytes of device workspace a call with this many latitudes and members needs. */
+size_t speedy_workspace_bytes(int32_t nlat, int32_t nmember);
+
+/* Zero a fresh workspace once (a fill kernel on `stream`); later calls leave it
+ * ready for the next call. */
+int speedy_workspace_init(void *workspace, size_t workspace_bytes, void *stream);
+
+/* Rows of the diagnostic output for kx levels, and their names as a
+ * comma-separated list ("name" for a 2-D row, "name[k]" rows for 3-D). */
+int speedy_diag_rows(int32_t kx);
+int speedy_diag_names(int32_t kx, char *buf, size_t buf_len);
+
+/* Forward: PhysicsTendency of SpeedyPhysics.compute_tendencies
+ * (speedy_physics.py:88-127) for nmember stacked states.
+ * level_set < 0: the Q1 cloud-top level set (shortwave_radiation.py:63-70) is
+ * computed per member over all its columns; >= 0: used as given (bit k =
+ * level k), for parity tests.  Stream-ordered; no allocation. */
+int speedy_fwd(const SpeedyFields *fields, const SpeedyVertical *vert, const SpeedyParams *params,
+               const SpeedyDate *date, int32_t nlon, int32_t nlat, int32_t nmember, uint32_t flags,
+               uint32_t term_mask, int32_t level_set, void *workspace, size_t workspace_bytes,
+               void *stream);
+
+/* Fused forward + VJP: the forward tendencies (as speedy_fwd) and the
+ * cotangent-weighted gradients w.r.t. T, q, u, v, ps and five parameters.
+ * Replaces jax.vjp of compute_tendencies (speedy_physics.py:88-127). */
+int speedy_fwd_vjp(const SpeedyFields *fields, const SpeedyAdjoint *adj, const SpeedyVertical *vert,
+                   const SpeedyParams *params, const SpeedyDate *date, int32_t nlon, int32_t nlat,
+                   int32_t nmember, uint32_t flags, int32_t level_set, void *workspace,
+                   size_t workspace_bytes, void *stream);
+
+/* Level set S of the last call's members (bit k = level k), copied to host. */
+int speedy_read_level_sets(const void *workspace, size_t workspace_bytes, int32_t nmember,
+                           uint32_t *out, void *stream);
+
+/* Kernel timers (no counterpart in the reference; for measurement).  While
+ * enabled, every call records HIP events on its stream before and after each
+ * of its kernels (up to `capacity` calls, then recording stops).
+ * speedy_timer_read synchronizes on the events and returns, per kernel kind
+ * (0 = level-set/latitude-table kernel, 1 = forward kernel, 2 = forward+VJP
+ * kernel), the summed milliseconds and launch count; it then clears them. */
+int speedy_timer_enable(int32_t capacity);
+int speedy_timer_disable(void);
+int speedy_timer_read(float *ms_per_kind, int32_t *launches_per_kind, int32_t nkinds);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char *speedy_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPEEDY_ABI_H */

@@ -1,0 +1,262 @@
+"""SpeedyPhysics on MI355X: drop-in for jcm/physics/speedy/speedy_physics.py:34-133.
+
+``SpeedyPhysics.compute_tendencies(state, forcing, geometry, date)`` keeps the
+reference signature and return value ``(PhysicsTendency, data)``
+(physics_interface.py:138); the eleven terms run fused in HIP kernels
+(libspeedy_hip.so, C ABI include/speedy_abi.h).  ``data`` is the PhysicsData
+subset the kernel writes when the object is built with
+``with_diagnostics=True`` (otherwise None, which avoids ~560 B/column of
+writes).  There is no CPU path: without the library or a GPU the call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from types import SimpleNamespace
+from typing import Any, Tuple
+
+import numpy as np
+import torch
+
+from . import _native
+from .date import DateData
+from .forcing import FORCING_FIELDS, ForcingData, day_slice
+from .geometry import Geometry
+from .params import VJP_PARAMS, Parameters
+from .physics_interface import Physics, PhysicsState, PhysicsTendency
+
+STATE_FIELDS = ("u_wind", "v_wind", "temperature", "specific_humidity", "geopotential")
+
+
+def _device(x, dev) -> torch.Tensor:
+    """float32 contiguous tensor on `dev`; tensors already there are used as is."""
+    if isinstance(x, torch.Tensor):
+        if x.device == dev and x.dtype == torch.float32 and x.is_contiguous():
+            return x
+        return x.to(device=dev, dtype=torch.float32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x, np.float32), device=dev)
+
+
+class PhysicsData:
+    """The PhysicsData subset written by the kernel, viewed per struct:
+    ``data.shortwave_rad.rsds``, ``data.mod_radcon.tau2`` (kx, ix, il, 4) etc.
+    Field names are those of jcm/physics/speedy/physics_data.py."""
+
+    def __init__(self, diag: torch.Tensor, names, kx: int, hshape):
+        self._diag = diag          # (rows, ncol) or (nmember, rows, ncol)
+        self._names = names
+        self._kx = kx
+        self._hshape = tuple(hshape)
+        self._index = {n: i for i, n in enumerate(names)}
+
+    def _row(self, name):
+        i = self._index[name]
+        d = self._diag[..., i, :]
+        return d.reshape(d.shape[:-1] + self._hshape)
+
+    def to_dict(self) -> dict:
+        """Flat {"struct.field[.channel]": array} like Physics.data_struct_to_dict."""
+        out = {}
+        for n in self._names:
+            if n.endswith("]"):
+                base = n[:n.index("[")]
+                if base not in out:
+                    out[base] = torch.stack([self._row(f"{base}[{k}]") for k in range(self._kx)], dim=-3)
+            else:
+                out[n] = self._row(n)
+        return out
+
+    def __getattr__(self, group):
+        if group.startswith("_"):
+            raise AttributeError(group)
+        d = self.to_dict()
+        sub = {}
+        for key, val in d.items():
+            parts = key.split(".")
+            if parts[0] != group:
+                continue
+            if len(parts) == 2:
+                sub[parts[1]] = val
+            else:
+                sub.setdefault(parts[1], {})[int(parts[2])] = val
+        if not sub:
+            raise AttributeError(group)
+        ns = {}
+        for k, v in sub.items():
+            ns[k] = torch.stack([v[i] for i in sorted(v)], dim=-1) if isinstance(v, dict) else v
+        return SimpleNamespace(**ns)
+
+
+class SpeedyPhysics(Physics):
+    """SPEEDY physics (speedy_physics.py:34-133) fused into gfx950 kernels."""
+
+    def __init__(self, parameters: Parameters | None = None, checkpoint_terms: bool = True,
+                 with_diagnostics: bool = False, device: str | torch.device | None = None):
+        self.parameters = parameters if parameters is not None else Parameters.default()
+        self.with_diagnostics = with_diagnostics
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device()) \
+            if torch.cuda.is_available() else None
+        self._lib = _native.load()
+        self._workspaces: dict = {}
+        self._consts: dict = {}
+        self._abi_params = None
+        self._abi_params_src = None
+
+    # ------------------------------------------------------------------ helpers
+    def _require_gpu(self):
+        if self.device is None or self.device.type != "cuda" or not torch.cuda.is_available():
+            raise _native.SpeedyError("SpeedyPhysics needs a ROCm GPU; there is no CPU path")
+
+    def _params(self) -> _native.SpeedyParams:
+        if self._abi_params is None or self._abi_params_src != repr(self.parameters):
+            self._abi_params = self.parameters.to_abi()
+            self._abi_params_src = repr(self.parameters)
+        return self._abi_params
+
+    def workspace(self, nlat: int, nmember: int) -> torch.Tensor:
+        key = (nlat, nmember)
+        ws = self._workspaces.get(key)
+        if ws is None:
+            nbytes = self._lib.speedy_workspace_bytes(nlat, nmember)
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            _native.check(self._lib.speedy_workspace_init(ws.data_ptr(), nbytes, self._stream()),
+                          "speedy_workspace_init")
+            self._workspaces[key] = ws
+        return ws
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _const(self, arr) -> torch.Tensor:
+        """Device copy of a geometry/forcing array, cached while the array lives."""
+        if isinstance(arr, torch.Tensor):
+            return _device(arr, self.device)
+        key = id(arr)
+        hit = self._consts.get(key)
+        if hit is not None and hit[0] is arr:
+            return hit[1]
+        t = _device(arr, self.device)
+        if len(self._consts) > 256:
+            self._consts.clear()
+        self._consts[key] = (arr, t)
+        return t
+
+    def _fields(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry, date: DateData,
+                keep: list):
+        kx, ix, il = geometry.nodal_shape
+        t = _device(state.temperature, self.device)
+        if t.shape[-3:] != (kx, ix, il):
+            raise ValueError(f"state shape {tuple(t.shape)} does not match geometry {geometry.nodal_shape}")
+        nmember = 1 if t.dim() == 3 else int(np.prod(t.shape[:-3]))
+        f = _native.SpeedyFields()
+        names = {"u_wind": "u", "v_wind": "v", "temperature": "t", "specific_humidity": "q", "geopotential": "phi"}
+        for src, dst in names.items():
+            x = _device(getattr(state, src), self.device)
+            if x.shape != t.shape:
+                raise ValueError(f"{src} has shape {tuple(x.shape)}, expected {tuple(t.shape)}")
+            keep.append(x)
+            setattr(f, dst, x.data_ptr())
+        ps = _device(state.normalized_surface_pressure, self.device)
+        if ps.numel() != nmember * ix * il:
+            raise ValueError("normalized_surface_pressure has the wrong size")
+        keep.append(ps)
+        f.ps = ps.data_ptr()
+        day = date.model_day()
+        fmap = dict(zip(FORCING_FIELDS, ("alb0", "sice_am", "snowc_am", "soilw_am", "stl_am", "sst")))
+        for src, dst in fmap.items():
+            arr = getattr(forcing, src)
+            if isinstance(arr, np.ndarray):
+                arr = day_slice(arr, day)
+                if arr.shape != (ix, il):
+                    arr = np.broadcast_to(arr, (ix, il))
+            elif arr.dim() > 2:
+                arr = arr[:, :, min(max(day, 0), arr.shape[2] - 1)]
+            x = self._const(arr)
+            keep.append(x)
+            setattr(f, dst, x.data_ptr())
+        for src in ("fmask", "orog", "phis0", "sia", "coa"):
+            x = self._const(getattr(geometry, src))
+            keep.append(x)
+            setattr(f, src, x.data_ptr())
+        return f, t, nmember
+
+    # ------------------------------------------------------------------ API
+    def compute_tendencies(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
+                           date: DateData, *, term_mask: int = _native.TERMS_ALL,
+                           level_set: int = -1) -> Tuple[PhysicsTendency, Any]:
+        """SpeedyPhysics.compute_tendencies (speedy_physics.py:88-127).
+
+        term_mask: bit i keeps term i's tendency in the sum (speedy_physics.py:67-80 order);
+        level_set: >= 0 overrides the Q1 cloud-top level set (parity tests)."""
+        self._require_gpu()
+        keep: list = []
+        f, t, nmember = self._fields(state, forcing, geometry, date, keep)
+        kx, ix, il = geometry.nodal_shape
+        out = [torch.empty_like(t) for _ in range(4)]
+        f.du, f.dv, f.dt, f.dq = (o.data_ptr() for o in out)
+        diag = None
+        if self.with_diagnostics:
+            rows = self._lib.speedy_diag_rows(kx)
+            diag = torch.empty(t.shape[:-3] + (rows, ix * il), dtype=torch.float32, device=self.device)
+            f.diag = diag.data_ptr()
+        else:
+            f.diag = None
+        ws = self.workspace(il, nmember)
+        flags = _native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0
+        rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(geometry.to_abi()), ctypes.byref(self._params()),
+                                  ctypes.byref(date.to_abi()), ix, il, nmember, flags, term_mask, level_set,
+                                  ws.data_ptr(), ws.numel(), self._stream())
+        _native.check(rc, "speedy_fwd")
+        data = PhysicsData(diag, _native.diag_names(kx), kx, (ix, il)) if diag is not None else None
+        return PhysicsTendency(*out), data
+
+    def compute_tendencies_vjp(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
+                               date: DateData, cotangent: PhysicsTendency, *, level_set: int = -1):
+        """Fused forward + VJP of compute_tendencies: returns (tendency, state_grads, param_grads)
+        where state_grads is a PhysicsState of gradients (geopotential entry None) and
+        param_grads maps the five VJP parameters (params.VJP_PARAMS) to per-member sums."""
+        self._require_gpu()
+        keep: list = []
+        f, t, nmember = self._fields(state, forcing, geometry, date, keep)
+        kx, ix, il = geometry.nodal_shape
+        out = [torch.empty_like(t) for _ in range(4)]
+        f.du, f.dv, f.dt, f.dq = (o.data_ptr() for o in out)
+        f.diag = None
+        adj = _native.SpeedyAdjoint()
+        cts = []
+        for name, src in zip(("ct_du", "ct_dv", "ct_dt", "ct_dq"),
+                             ("u_wind", "v_wind", "temperature", "specific_humidity")):
+            x = _device(getattr(cotangent, src), self.device)
+            if x.shape != t.shape:
+                raise ValueError(f"cotangent {src} has the wrong shape")
+            cts.append(x)
+            setattr(adj, name, x.data_ptr())
+        grads = [torch.empty_like(t) for _ in range(4)]
+        g_ps = torch.empty(t.shape[:-3] + (ix, il), dtype=torch.float32, device=self.device)
+        g_par = torch.empty(t.shape[:-3] + (len(VJP_PARAMS),), dtype=torch.float32, device=self.device)
+        adj.g_u, adj.g_v, adj.g_t, adj.g_q = (g.data_ptr() for g in grads)
+        adj.g_ps = g_ps.data_ptr()
+        adj.g_params = g_par.data_ptr()
+        ws = self.workspace(il, nmember)
+        flags = _native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0
+        rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(geometry.to_abi()),
+                                      ctypes.byref(self._params()), ctypes.byref(date.to_abi()), ix, il, nmember,
+                                      flags, level_set, ws.data_ptr(), ws.numel(), self._stream())
+        _native.check(rc, "speedy_fwd_vjp")
+        sg = PhysicsState(grads[0], grads[1], grads[2], grads[3], None, g_ps)
+        pg = {f"{g}.{n}": g_par[..., i] for i, (g, n) in enumerate(VJP_PARAMS)}
+        return PhysicsTendency(*out), sg, pg
+
+    def level_sets(self, nlat: int, nmember: int = 1) -> list:
+        """Q1 level set of each member in the last call with this (nlat, nmember)."""
+        ws = self.workspace(nlat, nmember)
+        buf = (ctypes.c_uint32 * nmember)()
+        _native.check(self._lib.speedy_read_level_sets(ws.data_ptr(), ws.numel(), nmember, buf, self._stream()),
+                      "speedy_read_level_sets")
+        return list(buf)
+
+    def get_empty_data(self, geometry: Geometry):
+        """speedy_physics.py:129-133: an all-zero data struct for accumulating averages."""
+        kx, ix, il = geometry.nodal_shape
+        rows = self._lib.speedy_diag_rows(kx)
+        z = torch.zeros((rows, ix * il), dtype=torch.float32, device=self.device)
+        return PhysicsData(z, _native.diag_names(kx), kx, (ix, il))

@@ -1,0 +1,88 @@
+// column_host.cpp -- TEST HARNESS ONLY: the kernel's column routine
+// (jax-gcm_amd/csrc/speedy_column.hpp) compiled for the host with g++, so the
+// CPU test suite can check the device code's logic against the oracle without
+// a GPU.  It is never loaded by the library or the product path.
+#include <stdint.h>
+#include <string.h>
+
+#include "../../jax-gcm_amd/csrc/speedy_column.hpp"
+
+using namespace speedy;
+
+template <int KX>
+static void run(const float* const* in, float* const* out, const SpeedyVertical* v, const SpeedyParams* p,
+                const SpeedyDate* d, int nlon, int nlat, uint32_t flags, uint32_t term_mask, int32_t level_set,
+                uint32_t* s_out) {
+  const int64_t ncol = (int64_t)nlon * nlat;
+  // in: u v t q phi ps alb0 sice snowc soilw stl sst fmask orog phis0 sia coa
+  const float *u = in[0], *vv = in[1], *t = in[2], *q = in[3], *phi = in[4], *ps = in[5];
+  float lat[6][1024];
+  for (int j = 0; j < nlat && j < 1024; ++j) {
+    float z[6];
+    zonal_fields(d->tyear, in[15][j], in[16][j], z);
+    for (int i = 0; i < 6; ++i) lat[i][j] = z[i];
+  }
+  Uniforms<KX> un;
+  make_uniforms<KX>(*p, *v, *d, un);
+  int32_t ms = d->model_step % 3;
+  if (ms < 0) ms += 3;
+  flags |= (ms == 0) ? (1u << 16) : 0u;
+  auto load = [&](int64_t c, ColumnIn<KX>& col) {
+    for (int k = 0; k < KX; ++k) {
+      col.T[k] = t[k * ncol + c]; col.Q[k] = q[k * ncol + c]; col.PHI[k] = phi[k * ncol + c];
+    }
+    col.u1 = u[(KX - 1) * ncol + c]; col.v1 = vv[(KX - 1) * ncol + c]; col.ps = ps[c];
+    col.alb0 = in[6][c]; col.sice = in[7][c]; col.snowc_am = in[8][c]; col.soilw = in[9][c];
+    col.stl = in[10][c]; col.sst = in[11][c]; col.fmask = in[12][c]; col.orog = in[13][c]; col.phis0 = in[14][c];
+    const int j = (int)(c % nlat);
+    col.fsol = lat[0][j]; col.ozupp = lat[1][j]; col.ozone = lat[2][j]; col.zenit = lat[3][j];
+    col.stratz = lat[4][j]; col.sqcoa = lat[5][j];
+  };
+  uint32_t S = 0;
+  for (int64_t c = 0; c < ncol; ++c) {
+    ColumnIn<KX> col;
+    load(c, col);
+    int icl = cloud_top<KX>(col, *p, *v, un);
+    icl = icl < 0 ? 0 : (icl > KX - 1 ? KX - 1 : icl);
+    S |= 1u << icl;
+  }
+  *s_out = S;
+  if (level_set >= 0) S = (uint32_t)level_set;
+  for (int64_t c = 0; c < ncol; ++c) {
+    ColumnIn<KX> col;
+    load(c, col);
+    PhaseA<KX> a;
+    phase_a<KX>(col, *p, *v, un, a);
+    PhaseB<KX> b;
+    phase_b<KX, true>(col, a, S, *p, *v, un, flags, b);
+    for (int k = 0; k < KX; ++k) {
+      float tt = 0.f, qq = 0.f, du = 0.f, dv = 0.f;
+      if (term_mask & (1u << SPEEDY_TERM_CONVECTION)) { tt = tt + a.tconv[k]; qq = qq + a.qconv[k]; }
+      if (term_mask & (1u << SPEEDY_TERM_CONDENSATION)) { tt = tt + a.tlsc[k]; qq = qq + a.qlsc[k]; }
+      if (term_mask & (1u << SPEEDY_TERM_SHORTWAVE)) tt = tt + b.tsw[k];
+      if (k == KX - 1 && (term_mask & (1u << SPEEDY_TERM_SURFACE))) {
+        du = du + b.du1; dv = dv + b.dv1; tt = tt + b.dts1; qq = qq + b.dqs1;
+      }
+      if (term_mask & (1u << SPEEDY_TERM_LW_UP)) tt = tt + b.tlw[k];
+      if (term_mask & (1u << SPEEDY_TERM_VDIFF)) { tt = tt + b.tvd[k]; qq = qq + b.qvd[k]; }
+      out[0][k * ncol + c] = du; out[1][k * ncol + c] = dv; out[2][k * ncol + c] = tt; out[3][k * ncol + c] = qq;
+    }
+    // a few diagnostics: iptop, icltop, cbmf, precnv, precls, cloudc, clstr, rsds, rlds, tskin
+    float* dg = out[4];
+    const float vals[] = {(float)a.iptop, (float)a.icltop, a.cbmf, a.precnv, a.precls, a.cloudc, a.clstr,
+                          b.rsds, b.rlds, b.tskin};
+    for (int i = 0; i < 10; ++i) dg[i * ncol + c] = vals[i];
+  }
+}
+
+extern "C" int column_host_fwd(int kx, const float* const* in, float* const* out, const SpeedyVertical* v,
+                               const SpeedyParams* p, const SpeedyDate* d, int nlon, int nlat, uint32_t flags,
+                               uint32_t term_mask, int32_t level_set, uint32_t* s_out) {
+  if (nlat > 1024) return 1;
+  switch (kx) {
+    case 7: run<7>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
+    case 8: run<8>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
+    case 16: run<16>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
+    default: return 2;
+  }
+}

@@ -1,0 +1,71 @@
+"""The C ABI library loads, exports every symbol include/speedy_abi.h declares, and
+rejects bad arguments before any GPU work (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from jcm_amd import _native
+from jcm_amd.geometry import Geometry
+from jcm_amd.params import Parameters
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "speedy_abi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(speedy_\w+)\s*\(", text)))
+
+
+def test_library_exports_declared_symbols():
+    lib = _native.load()
+    names = declared_functions()
+    assert "speedy_fwd" in names and "speedy_fwd_vjp" in names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(_native.SpeedyParams) == 4 * (59 + 4)
+    assert ctypes.sizeof(_native.SpeedyVertical) == 4 * (1 + 17 + 5 * 16 + 32)
+    assert ctypes.sizeof(_native.SpeedyDate) == 16
+    assert ctypes.sizeof(_native.SpeedyFields) == 8 * len(_native.FIELD_NAMES)
+
+
+def test_workspace_and_diag_queries():
+    lib = _native.load()
+    assert lib.speedy_abi_version() == 1
+    assert lib.speedy_workspace_bytes(128, 1) > 0
+    assert lib.speedy_workspace_bytes(0, 1) == 0
+    for kx in (7, 8, 16):
+        names = _native.diag_names(kx)
+        assert len(names) == lib.speedy_diag_rows(kx)
+        assert "shortwave_rad.rsds" in names and f"humidity.qsat[{kx - 1}]" in names
+    assert lib.speedy_diag_rows(9) < 0
+
+
+def _call(kx=8, nlon=4, nlat=2, nmember=1, flags=0, term_mask=_native.TERMS_ALL, level_set=-1, ws=1 << 20,
+          null_field=None):
+    lib = _native.load()
+    f = _native.SpeedyFields()
+    for i, n in enumerate(_native.FIELD_NAMES):
+        if n != "diag" and n != null_field:
+            setattr(f, n, 0x1000 + 64 * i)   # never dereferenced: validation fails first
+    g = Geometry.from_grid_shape((64, 32), num_levels=8).to_abi()
+    g.kx = kx
+    p = Parameters.default().to_abi()
+    d = _native.SpeedyDate()
+    return lib.speedy_fwd(ctypes.byref(f), ctypes.byref(g), ctypes.byref(p), ctypes.byref(d), nlon, nlat, nmember,
+                          flags, term_mask, level_set, ctypes.c_void_p(0x2000 if ws else 0), ws, None)
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(kx=9), 2), (dict(nlon=0), 1), (dict(nmember=0), 1), (dict(flags=4), 1),
+    (dict(term_mask=1 << 12), 1), (dict(level_set=1 << 9), 1), (dict(ws=0), 3), (dict(ws=16), 3),
+    (dict(null_field="t"), 1), (dict(null_field="coa"), 1)])
+def test_argument_errors(kw, code):
+    rc = _call(**kw)
+    assert rc == code
+    assert _native.load().speedy_last_error().decode()

@@ -1,0 +1,101 @@
+"""The kernel's column routine (speedy_column.hpp), built for the host, against the
+oracle on the same inputs: checks the device code's logic on the CPU.
+
+Tolerance: ||F - F_ref||_inf / ||F_ref||_inf <= 1e-5 per tendency field against
+the float32 oracle (2e-5 at 16 levels, where the float32 oracle itself is
+1.2e-5 from float64 because of cancellation in the convective flux
+differences); discrete decisions (iptop, icltop, level set S) must be equal.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+from jcm_amd import Parameters, synthetic
+
+FIELDS = ("u_wind", "v_wind", "temperature", "specific_humidity")
+
+
+def wet(case, seed, hi=1.15):
+    """RH ~ U(0.5, hi): condensation at many levels -> heterogeneous cloud tops (Q1)."""
+    state, forcing, geometry, date = case
+    rng = np.random.default_rng(seed)
+    rh = rng.uniform(0.5, hi, state.temperature.shape)
+    q = rh * synthetic.qsat_np(state.temperature, state.normalized_surface_pressure[None],
+                               geometry.fsg[:, None, None].astype(np.float64))
+    return (state.copy(specific_humidity=q.astype(np.float32)), forcing, geometry, date)
+
+
+CASES = {
+    "T31_aqua": dict(res=(96, 48), kx=8, land=False, wet=False),
+    "T21_land": dict(res=(64, 32), kx=8, land=True, wet=False),
+    "T21_wet": dict(res=(64, 32), kx=8, land=True, wet=True),
+    "T21_wet_kx16": dict(res=(64, 32), kx=16, land=True, wet=True),
+    "T21_wet_kx7": dict(res=(64, 32), kx=7, land=True, wet=True),
+    "T21_wet_swoff": dict(res=(64, 32), kx=8, land=True, wet=True, step=1),
+    "T21_wet_nofluxland": dict(res=(64, 32), kx=8, land=True, wet=True, lfluxland=False),
+}
+
+
+def make(name, seed=3):
+    c = CASES[name]
+    case = synthetic.make_case(*c["res"], c["kx"], seed=seed, land=c["land"])
+    if c["wet"]:
+        case = wet(case, seed + 2)
+    st, fo, ge, da = case
+    if "step" in c:
+        da = da.copy(model_step=c["step"])
+    if "lfluxland" in c:
+        fo = fo.copy(lfluxland=c["lfluxland"])
+    return st, fo, ge, da
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_host_column_matches_oracle(name):
+    case = make(name)
+    p = synthetic.draw_vjp_parameters(2) if CASES[name]["wet"] else Parameters.default()
+    outs, diag, S = H.run_host_column(case, p)
+    tend, data, S_ref = H.run_oracle(*case, params=p)
+    assert S == S_ref
+    np.testing.assert_array_equal(diag["iptop"], data.convection.iptop)
+    np.testing.assert_array_equal(diag["icltop"], data.shortwave_rad.icltop)
+    tol = 2e-5 if CASES[name]["kx"] == 16 else H.REL_TOL
+    for f, a in zip(FIELDS, outs):
+        assert H.rel_inf(a, getattr(tend, f)) <= tol, f
+    for k, ref in (("cbmf", data.convection.cbmf), ("precnv", data.convection.precnv),
+                   ("precls", data.condensation.precls), ("cloudc", data.shortwave_rad.cloudc),
+                   ("clstr", data.shortwave_rad.cloudstr), ("rsds", data.shortwave_rad.rsds),
+                   ("rlds", data.surface_flux.rlds), ("tskin", data.surface_flux.tskin)):
+        assert H.rel_inf(diag[k], ref) <= 1e-5, k
+
+
+@pytest.mark.parametrize("term", [3, 4, 6, 8, 9, 10])
+def test_host_column_single_terms(term):
+    """Each term's tendency alone (term_mask) against the oracle's.  A single term's
+    field can be small next to its float32 rounding (e.g. shortwave at level 0 is a
+    difference of two ~1300 W/m^2 fluxes), so the bound is: within 2e-5 of the float32
+    oracle and as close to the float64 oracle as the float32 oracle is (or 1e-5)."""
+    case = make("T21_wet")
+    p = Parameters.default()
+    outs, _, _ = H.run_host_column(case, p, term_mask=1 << term)
+    tend, _, _ = H.run_oracle(*case, params=p, term_mask=1 << term)
+    t64, _, _ = H.run_oracle(*case, params=p, term_mask=1 << term, dtype=np.float64)
+    for f, a in zip(FIELDS, outs):
+        ref, ref64 = getattr(tend, f), getattr(t64, f)
+        if np.max(np.abs(ref)) == 0:
+            assert np.max(np.abs(a)) == 0, f
+        else:
+            assert H.rel_inf(a, ref) <= 2e-5, f
+            assert H.rel_inf(a, ref64) <= max(1e-5, 2 * H.rel_inf(ref, ref64)), f
+
+
+def test_host_column_level_set_override():
+    """With S given, the tendencies equal the oracle's with the same S."""
+    case = make("T21_wet")
+    p = Parameters.default()
+    S = 0b0101010
+    outs, _, _ = H.run_host_column(case, p, level_set=S)
+    tend, _, S_used = H.run_oracle(*case, params=p, level_set=S)
+    assert S_used == S
+    assert H.rel_inf(outs[2], tend.temperature) <= H.REL_TOL
+    ref_free, _, _ = H.run_oracle(*case, params=p)
+    assert H.rel_inf(tend.temperature, ref_free.temperature) > 1e-4   # S matters
