@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
-                    help="run --steps plain calls and exit (for rocprofv3)")
+                    help="replay --steps calls, run a copy of --copy-mib MiB, and exit (for rocprofv3)")
+    ap.add_argument("--copy-mib", type=int, default=256)
     return ap.parse_args()
 
 
@@ -93,33 +94,46 @@ def main():
     phys = SpeedyPhysics(device=dev)
     lib = _native.load()
 
-    if args.profile_only:
-        for _ in range(args.steps):
-            phys.compute_tendencies(*case)
+    def graph_of(calls):
+        stream = torch.cuda.Stream(device=dev)
+        stream.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(stream):
+            phys.compute_tendencies(*case)     # warm the side stream
         torch.cuda.synchronize()
-        print(f"profile run: {args.steps} forward calls at T85", flush=True)
-        return
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            for _ in range(calls):
+                phys.compute_tendencies(*case)
+        torch.cuda.synchronize()
+        return g
 
     for _ in range(args.warmup):
         phys.compute_tendencies(*case)
     torch.cuda.synchronize()
-
-    # ---- timed region: graph replays of G calls each
     G = max(1, min(args.graph_calls, args.steps))
     reps = max(1, (args.steps + G - 1) // G)
     steps = reps * G
-    stream = torch.cuda.Stream(device=dev)
-    stream.wait_stream(torch.cuda.current_stream(dev))
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.stream(stream):
-        phys.compute_tendencies(*case)     # warm the side stream
-    torch.cuda.synchronize()
-    with torch.cuda.graph(graph, stream=stream):
-        for _ in range(G):
-            phys.compute_tendencies(*case)
-    torch.cuda.synchronize()
+    graph = graph_of(G)
     graph.replay()
     torch.cuda.synchronize()
+
+    if args.profile_only:
+        for _ in range(reps):
+            graph.replay()
+        torch.cuda.synchronize()
+        # A copy of known size (torch.neg: reads and writes copy_mib MiB each) whose
+        # FETCH_SIZE / WRITE_SIZE counters calibrate the counter units per byte.
+        n = (args.copy_mib << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(f"profile run: {steps} forward calls at T85 replayed from graphs of {G}; "
+              f"3 x neg of {args.copy_mib} MiB", flush=True)
+        return
+
+    # ---- timed region: K steps replayed from graphs, bracketed by synchronize
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -132,15 +146,27 @@ def main():
     ms_per_step = ms / steps
     value = ncol / (ms_per_step * 1e-3)
 
-    # ---- per-kernel device time (HIP events around each launch)
-    lib.speedy_timer_enable(2 * args.timer_steps + 8)
-    for _ in range(args.timer_steps):
+    # ---- kernel launches per step (library timer on plain calls, counts only)
+    lib.speedy_timer_enable(64)
+    for _ in range(4):
         phys.compute_tendencies(*case)
     torch.cuda.synchronize()
     tm = _native.timer_read()
     lib.speedy_timer_disable()
-    fwd_ms = tm["fwd"][0] / max(1, tm["fwd"][1])
-    lvl_ms = tm["levelset"][0] / max(1, tm["levelset"][1])
+    launches = {k: v[1] // 4 for k, v in tm.items() if v[1]}
+    # The dominant (forward) kernel's average launch time inside the timed region:
+    # HIP events around the replays of the same graph, per launch.
+    fwd_ms = ms_per_step if launches.get("levelset", 0) == 0 else None
+    if fwd_ms is None:
+        # two-launch path: time the forward launches alone is not separable from the
+        # graph; fall back to the event timer around each launch on plain calls
+        lib.speedy_timer_enable(2 * 200 + 8)
+        for _ in range(200):
+            phys.compute_tendencies(*case)
+        torch.cuda.synchronize()
+        tm = _native.timer_read()
+        lib.speedy_timer_disable()
+        fwd_ms = tm["fwd"][0] / max(1, tm["fwd"][1])
     bytes_launch = FWD_BYTES_PER_COL(kx) * ncol
     achieved = bytes_launch / (fwd_ms * 1e-3) / 1e9
 
@@ -158,8 +184,8 @@ def main():
                    "columns": ncol, "levels": kx, "pass": "fwd", "graph_calls": G},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "fwd_kernel<8,false>", "kernel_ms": fwd_ms,
-                     "algorithmic_bytes_per_launch": bytes_launch, "levelset_kernel_ms": lvl_ms},
+                     "kernel": "speedy::fwd_kernel<8, false, true>", "kernel_ms": fwd_ms,
+                     "launches_per_step": launches, "algorithmic_bytes_per_launch": bytes_launch},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case_np, args.cpu_seconds)

@@ -120,12 +120,17 @@ typedef struct {
 /* Version of this ABI (SPEEDY_ABI_VERSION). */
 int speedy_abi_version(void);
 
-/* Bytes of device workspace a call with this many latitudes and members needs. */
-size_t speedy_workspace_bytes(int32_t nlat, int32_t nmember);
+/* Bytes of device workspace a call on this grid needs (0 if unsupported).  A
+ * workspace may be reused by later calls of the same or a smaller grid. */
+size_t speedy_workspace_bytes(int32_t nlon, int32_t nlat, int32_t nmember);
 
 /* Zero a fresh workspace once (a fill kernel on `stream`); later calls leave it
  * ready for the next call. */
 int speedy_workspace_init(void *workspace, size_t workspace_bytes, void *stream);
+
+/* Status of a workspace: bit 0 is set if a call's grid barrier gave up
+ * waiting (that call's results are invalid).  Reading clears it. */
+int speedy_workspace_status(void *workspace, size_t workspace_bytes, uint32_t *status, void *stream);
 
 /* Rows of the diagnostic output for kx levels, and their names as a
  * comma-separated list ("name" for a 2-D row, "name[k]" rows for 3-D). */

@@ -62,36 +62,21 @@ SPD_HD void radset4(float temp, float omeps, float fb[4]) {
   fb[0] = omeps * f0; fb[1] = omeps * f1; fb[2] = omeps * f2; fb[3] = omeps * f3;
 }
 
-// Per-latitude part of get_zonal_average_fields (shortwave_radiation.py:220-296)
-// including solar (:406-451).  out: fsol, ozupp, ozone, zenit, stratz, sqrt(coa).
-SPD_HD void zonal_fields(float tyear, float sia, float coa, float out[6]) {
-  const float asin1 = asinf(1.0f);
-  // solar(tyear, 4*solc)
-  const float pigr = 2.0f * asin1;
-  const float alpha_s = 2.0f * pigr * tyear;
-  const float ca1 = cosf(alpha_s), sa1 = sinf(alpha_s);
-  const float ca2 = ca1 * ca1 - sa1 * sa1;
-  const float sa2 = 2.0f * sa1 * ca1;
-  const float ca3 = ca1 * ca2 - sa1 * sa2;
-  const float sa3 = sa1 * ca2 + sa2 * ca1;
-  const float decl = 0.006918f - 0.399912f * ca1 + 0.070257f * sa1 - 0.006758f * ca2 +
-                     0.000907f * sa2 - 0.002697f * ca3 + 0.001480f * sa3;
-  const float fdis = 1.000110f + 0.034221f * ca1 + 0.001280f * sa1 + 0.000719f * ca2 + 0.000077f * sa2;
-  const float cdecl = cosf(decl), sdecl = sinf(decl);
-  const float tdecl = sdecl / cdecl;
-  const float csolp = (4.0f * 342.0f) / pigr;
+// Latitude-dependent part of solar and get_zonal_average_fields
+// (shortwave_radiation.py:267-287,446-450) for one column, from the date-only
+// uniforms.  out: fsol, ozupp, ozone, zenit, stratz, sqrt(coa).
+struct ZonalUniforms {
+  float sol_tdecl, sol_sdecl, sol_cdecl, sol_scale, oz_coz1, zen_c, zen_s;
+};
+SPD_HD void zonal_column(const ZonalUniforms& z, float sia, float coa, float out[6]) {
   // clip bounds +-(1-1e-9) are exactly +-1 in float32 (:446)
-  const float ch0 = fminf(fmaxf(-tdecl * sia / coa, -1.0f), 1.0f);
+  const float ch0 = fminf(fmaxf(-z.sol_tdecl * sia / coa, -1.0f), 1.0f);
   const float h0 = acosf(ch0);
   const float sh0 = sinf(h0);
-  const float topsr = csolp * fdis * (h0 * sia * sdecl + sh0 * coa * cdecl);
-  // zonal fields
-  const float alpha = 4.0f * asin1 * (tyear + (float)(10.0 / 365.0));
-  const float coz1 = fmaxf(0.0f, cosf(alpha));
-  const float rzen = -cosf(alpha) * 23.45f * asin1 / 90.0f;
+  const float topsr = z.sol_scale * (h0 * sia * z.sol_sdecl + sh0 * coa * z.sol_cdecl);
   const float flat2 = 1.5f * sia * sia - 0.5f;
-  const float ozone0 = (float)(0.4 * 0.020) * (1.0f + coz1 * sia + 1.8f * flat2);
-  const float zenit = 1.0f + 1.0f * sq(1.0f - (coa * cosf(rzen) + sia * sinf(rzen)));
+  const float ozone0 = (float)(0.4 * 0.020) * (1.0f + z.oz_coz1 * sia + 1.8f * flat2);
+  const float zenit = 1.0f + 1.0f * sq(1.0f - (coa * z.zen_c + sia * z.zen_s));
   out[0] = topsr;
   out[1] = topsr * (float)(0.5 * 0.020) * zenit;
   out[2] = topsr * ozone0 * zenit;
@@ -122,44 +107,122 @@ struct PhaseA {
   int icltop;
 };
 
-// Derived uniform constants that only depend on parameters and levels.
-template <int KX>
+// Uniform constants that depend only on parameters, levels and the date.  They
+// are computed once per call on the host (make_uniforms) in float32, in the
+// reference's operation order, and passed to the kernels by value.
 struct Uniforms {
-  float entr[KX];                 // entrainment profile (convection.py:136-138), levels 1..KX-2
-  float fm0, rdps;                // convection.py:141-142
-  float rtlsc, rhref[KX], dqmax[KX];  // large_scale_condensation.py:48,59-63
-  float ablco2;                   // speedy/forcing.py:33 (times co2_scale)
-  float wvi1[KX];
+  int32_t kx;
+  uint32_t compute_sw;            // set_physics_flags: model_step % nstrad == 0 (speedy_physics.py:26-27)
+  float wvi1[SPEEDY_KX_MAX];      // geometry.wvi[:, 1]
+  // convection (convection.py:136-142)
+  float entr[SPEEDY_KX_MAX], fm0, rdps;
+  // large-scale condensation (large_scale_condensation.py:48-63,77)
+  float rtlsc, rhref[SPEEDY_KX_MAX], dqmax[SPEEDY_KX_MAX], pfact[SPEEDY_KX_MAX];
+  // clouds (shortwave_radiation.py:348,387)
+  float rrcl, rgse;
+  // shortwave absorptivity of dry air + aerosol per level (shortwave_radiation.py:82,93)
+  float sw_abs1[SPEEDY_KX_MAX];
+  // longwave (speedy/forcing.py:33; shortwave_radiation.py:203; longwave_radiation.py:100,156)
+  float ablco2, eps1, omeps, refsfc, corlw;
+  // surface fluxes (surface_flux.py:75-76,101,135-137,149,154,253,311)
+  float esbc, ghum0, gtemp0, astab, rdth, rhdrag, cdl, cds, chlcp, chscp;
+  // vertical diffusion (vertical_diffusion.py:45-63,99-118)
+  float fshcq, fshcse, fvdiq2, drh0, fvdise;
+  float rsig[SPEEDY_KX_MAX], rsig1[SPEEDY_KX_MAX], vd_fq[SPEEDY_KX_MAX], vd_drh0[SPEEDY_KX_MAX];
+  uint32_t vd_on;                 // bit k: hsg[k+1] > 0.5
+  // date-only parts of solar and get_zonal_average_fields (shortwave_radiation.py:250-262,421-443)
+  ZonalUniforms zon;
 };
 
-template <int KX>
-SPD_HD void make_uniforms(const SpeedyParams& p, const SpeedyVertical& v, const SpeedyDate& d, Uniforms<KX>& u) {
+inline void make_uniforms(const SpeedyParams& p, const SpeedyVertical& v, const SpeedyDate& d, Uniforms& u) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  const int kx = v.kx;
+  u = Uniforms();
+  u.kx = kx;
+  int32_t ms = d.model_step % 3;
+  if (ms < 0) ms += 3;
+  u.compute_sw = ms == 0;
+  for (int k = 0; k < kx; ++k) u.wvi1[k] = v.wvi[k][1];
+  float e[SPEEDY_KX_MAX] = {0};
   float sentr = 0.0f;
-  float e[KX];
-#pragma unroll
-  for (int k = 0; k < KX; ++k) e[k] = 0.0f;
-#pragma unroll
-  for (int k = 1; k < KX - 1; ++k) {
+  for (int k = 1; k < kx - 1; ++k) {
     e[k] = sq(fmaxf(0.0f, v.fsg[k] - 0.5f));
-    sentr += e[k];
+    sentr = (k == 1) ? e[k] : sentr + e[k];
   }
   const float sc = p.entmax / sentr;
-#pragma unroll
-  for (int k = 0; k < KX; ++k) u.entr[k] = e[k] * sc;
-  u.fm0 = P0 * v.dhs[KX - 1] / (GRAV * p.trcnv * 3600.0f);
+  for (int k = 0; k < kx; ++k) u.entr[k] = e[k] * sc;
+  u.fm0 = P0 * v.dhs[kx - 1] / (GRAV * p.trcnv * 3600.0f);
   u.rdps = 2.0f / (1.0f - p.psmin);
   u.rtlsc = 1.0f / (p.trlsc * 3600.0f);
-#pragma unroll
-  for (int k = 0; k < KX; ++k) {
+  for (int k = 0; k < kx; ++k) {
     const float sig2 = sq(v.fsg[k]);
     u.rhref[k] = fmaxf(p.rhlsc + p.drhlsc * (sig2 - 1.0f), p.rhblsc);
     u.dqmax[k] = 10.0f * sig2 * u.rtlsc;
-    u.wvi1[k] = v.wvi[k][1];
+    u.pfact[k] = v.dhs[k] * PRG;
+    u.sw_abs1[k] = p.absdry + p.absaer * sq(v.fsg[k]);
   }
-  float ab = 6.0f * expf((float)p.increase_co2 * 0.005f *
-                         ((float)d.model_year + d.tyear - (float)p.co2_year_ref));
+  u.rrcl = 1.0f / (p.rhcl2 - p.rhcl1);
+  u.rgse = 1.0f / (p.gse_s1 - p.gse_s0);
+  float ab = 6.0f * expf((float)p.increase_co2 * 0.005f * ((float)d.model_year + d.tyear - (float)p.co2_year_ref));
   if (p.co2_scale != 1.0f) ab = ab * p.co2_scale;
   u.ablco2 = ab;
+  u.eps1 = p.epslw / (v.dhs[0] + v.dhs[1]);
+  u.omeps = 1.0f - p.epslw;
+  u.refsfc = 1.0f - p.emisfc;
+  u.corlw = p.epslw * p.emisfc;
+  u.esbc = p.emisfc * SBC;
+  u.ghum0 = 1.0f - p.fhum0;
+  u.gtemp0 = 1.0f - p.ftemp0;
+  u.astab = p.lscasym ? 0.5f : 1.0f;
+  u.rdth = p.fstab / p.dtheta;
+  u.rhdrag = 1.0f / (GRAV * p.hdrag);
+  u.cdl = p.drag_scale != 1.0f ? p.cdl * p.drag_scale : p.cdl;
+  u.cds = p.drag_scale != 1.0f ? p.cds * p.drag_scale : p.cds;
+  u.chlcp = p.chl * CP;
+  u.chscp = p.chs * CP;
+  const int nl1 = kx - 1;
+  const float cshc = v.dhs[kx - 1] / 3600.0f;
+  const float cvdi = (v.hsg[nl1] - v.hsg[1]) / (float)((nl1 - 1) * 3600.0);
+  u.fshcq = cshc / p.trshc;
+  u.fshcse = cshc / (p.trshc * CP);
+  const float fvdiq = cvdi / p.trvdi;
+  u.fvdise = cvdi / (p.trvds * CP);
+  u.fvdiq2 = fvdiq * v.hsg[nl1];
+  u.drh0 = p.rhgrad * (v.fsg[kx - 1] - v.fsg[nl1 - 1]);
+  {
+    const float asin1 = asinf(1.0f);
+    const float pigr = 2.0f * asin1;
+    const float alpha_s = 2.0f * pigr * d.tyear;
+    const float ca1 = cosf(alpha_s), sa1 = sinf(alpha_s);
+    const float ca2 = ca1 * ca1 - sa1 * sa1;
+    const float sa2 = 2.0f * sa1 * ca1;
+    const float ca3 = ca1 * ca2 - sa1 * sa2;
+    const float sa3 = sa1 * ca2 + sa2 * ca1;
+    const float decl = 0.006918f - 0.399912f * ca1 + 0.070257f * sa1 - 0.006758f * ca2 + 0.000907f * sa2 -
+                       0.002697f * ca3 + 0.001480f * sa3;
+    const float fdis = 1.000110f + 0.034221f * ca1 + 0.001280f * sa1 + 0.000719f * ca2 + 0.000077f * sa2;
+    u.zon.sol_cdecl = cosf(decl);
+    u.zon.sol_sdecl = sinf(decl);
+    u.zon.sol_tdecl = u.zon.sol_sdecl / u.zon.sol_cdecl;
+    u.zon.sol_scale = ((4.0f * 342.0f) / pigr) * fdis;
+    const float alpha = 4.0f * asin1 * (d.tyear + (float)(10.0 / 365.0));
+    u.zon.oz_coz1 = fmaxf(0.0f, cosf(alpha));
+    const float rzen = -cosf(alpha) * 23.45f * asin1 / 90.0f;
+    u.zon.zen_c = cosf(rzen);
+    u.zon.zen_s = sinf(rzen);
+  }
+  u.vd_on = 0;
+  for (int k = 0; k < kx; ++k) {
+    u.rsig[k] = 1.0f / v.dhs[k];
+    u.rsig1[k] = (k < kx - 1) ? 1.0f / (1.0f - v.hsg[k + 1]) : 0.0f;
+    if (k + 1 < kx) {
+      u.vd_fq[k] = fvdiq * v.hsg[k + 1];
+      u.vd_drh0[k] = p.rhgrad * (v.fsg[k + 1] - v.fsg[k]);
+      if (v.hsg[k + 1] > 0.5f) u.vd_on |= 1u << k;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -167,7 +230,7 @@ SPD_HD void make_uniforms(const SpeedyParams& p, const SpeedyVertical& v, const 
 // ---------------------------------------------------------------------------
 template <int KX>
 SPD_HD void phase_a(const ColumnIn<KX>& c, const SpeedyParams& p, const SpeedyVertical& v,
-                    const Uniforms<KX>& u, PhaseA<KX>& a) {
+                    const Uniforms& u, PhaseA<KX>& a) {
   // humidity (humidity.py:40-46)
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
@@ -310,7 +373,7 @@ SPD_HD void phase_a(const ColumnIn<KX>& c, const SpeedyParams& p, const SpeedyVe
     }
 #pragma unroll
     for (int k = 1; k < KX; ++k) {
-      const float t = v.dhs[k] * PRG * a.qlsc[k];
+      const float t = u.pfact[k] * a.qlsc[k];
       precls = (k == 1) ? t : precls + t;
     }
     a.precls = (0.0f - precls) * c.ps;
@@ -320,7 +383,7 @@ SPD_HD void phase_a(const ColumnIn<KX>& c, const SpeedyParams& p, const SpeedyVe
   // ---- clouds (shortwave_radiation.py:313-404)
   {
     a.gse = (a.se[KX - 2] - a.se[KX - 1]) / (c.PHI[KX - 2] - c.PHI[KX - 1]);
-    const float rrcl = 1.0f / (p.rhcl2 - p.rhcl1);
+    const float rrcl = u.rrcl;
     const bool m0 = a.rh[KX - 2] > p.rhcl1;
     const float cc0 = m0 ? a.rh[KX - 2] - p.rhcl1 : 0.0f;
     const int icl0 = m0 ? KX - 2 : KX;
@@ -346,7 +409,7 @@ SPD_HD void phase_a(const ColumnIn<KX>& c, const SpeedyParams& p, const SpeedyVe
     a.cloudc = cloudc;
     a.icltop = a.iptop < icl ? a.iptop : icl;
     a.qcloud = c.Q[KX - 2];
-    const float rgse = 1.0f / (p.gse_s1 - p.gse_s0);
+    const float rgse = u.rgse;
     const float fstab = fminf(fmaxf(rgse * (a.gse - p.gse_s0), 0.0f), 1.0f);
     float clstr = fstab * fmaxf(p.clsmax - 1.2f * cloudc, 0.0f);
     const float clstrl = fmaxf(clstr, p.clsminl) * a.rh[KX - 1];
@@ -358,7 +421,7 @@ SPD_HD void phase_a(const ColumnIn<KX>& c, const SpeedyParams& p, const SpeedyVe
 // convective fluxes.  Must agree exactly with phase_a's icltop.
 template <int KX>
 SPD_HD int cloud_top(const ColumnIn<KX>& c, const SpeedyParams& p, const SpeedyVertical& v,
-                     const Uniforms<KX>& u) {
+                     const Uniforms& u) {
   float qsat[KX], rh[KX], se[KX];
 #pragma unroll
   for (int k = 1; k < KX; ++k) {
@@ -421,33 +484,170 @@ SPD_HD int cloud_top(const ColumnIn<KX>& c, const SpeedyParams& p, const SpeedyV
 }
 
 // ---------------------------------------------------------------------------
-// Phase B results.
+// Phase B.  b1 needs only phase A and the column (albedos, longwave
+// transmissivity, longwave down, vertical diffusion); b2 needs the level set S
+// and the per-latitude tables (shortwave, surface fluxes, longwave up).
 // ---------------------------------------------------------------------------
 template <int KX>
 struct PhaseB {
-  float tsw[KX], tlw[KX], tvd[KX], qvd[KX];
-  float du1, dv1, dts1, dqs1;        // surface-flux tendencies at level KX-1
-  // diagnostics
+  // b1
   float albsfc, alb_l, alb_s, snowc;
+  float tauL[KX][4], st4a0[KX], st4a1[KX], dfl[KX], rlds;
+  float tvd[KX], qvd[KX];
+  // b2
+  float tsw[KX], tlw[KX];
+  float du1, dv1, dts1, dqs1;        // surface-flux tendencies at level KX-1
+  // diagnostics (b2)
   float dfabs_sw[KX], rsds, rsns, ftop_sw, stratc0, stratc1;
-  float st4a0[KX], st4a1[KX], tau[KX][4];
-  float dfabs_lw[KX], rlds, rlns, ftop_lw, flux_out[4];
+  float rlns, ftop_lw, flux_out[4];
   float ustr[3], vstr[3], shf[3], evap[3], rlus[3], hfluxn[2], tsfc, tskin, u0, v0, t0;
 };
 
-template <int KX, bool DIAG>
-SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, const SpeedyParams& p,
-                    const SpeedyVertical& v, const Uniforms<KX>& u, uint32_t flags, PhaseB<KX>& b) {
+// Transmissivities use the hardware exponential (v_exp_f32 on x*log2(e)):
+// relative error ~6e-8*|x| (< 2e-6 where the result matters), far inside the
+// 1e-5 parity budget; decision-bearing exponentials (qsat) stay libm-accurate.
+SPD_HD float exp_tr(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __expf(x);
+#else
+  return expf(x);
+#endif
+}
+
+template <int KX>
+SPD_HD void phase_b1(const ColumnIn<KX>& c, const PhaseA<KX>& a, const SpeedyParams& p, const SpeedyVertical& v,
+                     const Uniforms& u, PhaseB<KX>& b) {
   // ---- set_forcing albedos (speedy/forcing.py:26-30)
-  const float snowc = fminf(1.0f, c.snowc_am);
-  const float alb_l = c.alb0 + snowc * (p.albsn - c.alb0);
-  const float alb_s = p.albsea + c.sice * (p.albice - p.albsea);
-  const float albsfc = alb_s + c.fmask * (alb_l - alb_s);
-  if (DIAG) { b.snowc = snowc; b.alb_l = alb_l; b.alb_s = alb_s; b.albsfc = albsfc; }
+  b.snowc = fminf(1.0f, c.snowc_am);
+  b.alb_l = c.alb0 + b.snowc * (p.albsn - c.alb0);
+  b.alb_s = p.albsea + c.sice * (p.albice - p.albsea);
+  b.albsfc = b.alb_s + c.fmask * (b.alb_l - b.alb_s);
   const float ps = c.ps;
+  // ---- longwave transmissivity (shortwave_radiation.py:177-200)
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    float ab[4] = {p.ablwin, u.ablco2, p.ablwv1 * c.Q[k], p.ablwv2 * c.Q[k]};
+    if (k == 0) { ab[2] = 0.0f; ab[3] = 0.0f; }
+    if (k >= 2 && k <= KX - 2) {
+      ab[0] = ab[0] + ((k > a.icltop) ? a.cloudc * p.ablcl1 : a.cloudc * p.ablcl2);
+      const float ac2 = a.cloudc * p.ablcl2;
+      ab[2] = fmaxf(ab[2], ac2);
+      ab[3] = fmaxf(ab[3], ac2);
+    }
+#pragma unroll
+    for (int bnd = 0; bnd < 4; ++bnd) b.tauL[k][bnd] = exp_tr(-ab[bnd] * ps * v.dhs[k]);
+  }
+  // ---- longwave down (longwave_radiation.py:13-117)
+  {
+    float Th[KX];
+#pragma unroll
+    for (int k = 0; k < KX - 1; ++k) Th[k] = c.T[k] + u.wvi1[k] * (c.T[k + 1] - c.T[k]);
+    const float s0 = 0.75f * c.T[0] + 0.25f * Th[0];
+    const float s1 = 0.50f * c.T[1] + 0.25f * (Th[0] + Th[1]);
+    float grad[KX];
+#pragma unroll
+    for (int k = 2; k < KX - 1; ++k) grad[k] = 0.5f * fmaxf(Th[k] - Th[k - 1], 0.0f);
+    grad[KX - 1] = fmaxf(c.T[KX - 1] - Th[KX - 2], 0.0f);
+    b.st4a0[0] = SBC * pow4(s0); b.st4a0[1] = SBC * pow4(s1);
+    b.st4a1[0] = 0.0f; b.st4a1[1] = 0.0f;
+#pragma unroll
+    for (int k = 2; k < KX; ++k) {
+      const float st3a = SBC * pow3(c.T[k]);
+      b.st4a0[k] = st3a * c.T[k];
+      b.st4a1[k] = 4.0f * st3a * grad[k];
+    }
+    float Fd[4];
+    {
+      float fb[4];
+      radset4(c.T[0], u.omeps, fb);
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) {
+        const float em = 1.0f - b.tauL[0][bnd];
+        Fd[bnd] = bnd < 2 ? em * (fb[bnd] * (b.st4a0[0] + em * b.st4a1[0])) : 0.0f;
+      }
+    }
+    float Fprev = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
+    b.dfl[0] = -Fprev;
+#pragma unroll
+    for (int k = 1; k < KX; ++k) {
+      float fb[4];
+      radset4(c.T[k], u.omeps, fb);
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) {
+        const float em = 1.0f - b.tauL[k][bnd];
+        Fd[bnd] = b.tauL[k][bnd] * Fd[bnd] + em * (fb[bnd] * (b.st4a0[k] + em * b.st4a1[k]));
+      }
+      const float Fs = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
+      b.dfl[k] = -(Fs - Fprev);
+      Fprev = Fs;
+    }
+    float rlds = ((p.emisfc * Fd[0] + p.emisfc * Fd[1]) + p.emisfc * Fd[2]) + p.emisfc * Fd[3];
+    const float corlw = u.corlw * b.st4a0[KX - 1];
+    b.dfl[KX - 1] = b.dfl[KX - 1] + (-corlw);
+    b.rlds = rlds + corlw;
+  }
+  // ---- vertical diffusion (vertical_diffusion.py:10-135)
+  {
+    const int icnv = KX - a.iptop - 1;
+    constexpr int NL1 = KX - 1;
+    float tt[KX], qt[KX];
+#pragma unroll
+    for (int k = 0; k < KX; ++k) { tt[k] = 0.0f; qt[k] = 0.0f; }
+    const float dmse = a.se[KX - 1] - a.se[NL1 - 1] + ALHC * (c.Q[KX - 1] - a.qsat[NL1 - 1]);
+    const float drh = a.rh[KX - 1] - a.rh[NL1 - 1];
+    const float fcnv = (icnv > 0 && dmse >= 0.0f) ? p.redshc : 1.0f;
+    const float rsA = u.rsig[NL1 - 1], rsB = u.rsig[KX - 1];
+    if (dmse >= 0.0f) {
+      const float fluxse = fcnv * u.fshcse * dmse;
+      tt[NL1 - 1] = fluxse * rsA;
+      tt[KX - 1] = -fluxse * rsB;
+      if (drh >= 0.0f) {
+        const float fluxq = fcnv * u.fshcq * a.qsat[KX - 1] * drh;
+        qt[NL1 - 1] = fluxq * rsA;
+        qt[KX - 1] = -fluxq * rsB;
+      }
+    } else if (dmse < 0.0f && drh > u.drh0) {
+      const float fluxq = u.fvdiq2 * a.qsat[NL1 - 1] * drh;
+      qt[NL1 - 1] = fluxq * rsA;
+      qt[KX - 1] = -fluxq * rsB;
+    }
+    // moisture diffusion above the PBL (:99-118)
+    float fq[KX];
+#pragma unroll
+    for (int k = 2; k < KX - 2; ++k) {
+      const float d = a.rh[k + 1] - a.rh[k];
+      fq[k] = (d >= u.vd_drh0[k] && ((u.vd_on >> k) & 1u)) ? u.vd_fq[k] * a.qsat[k] * d : 0.0f;
+    }
+#pragma unroll
+    for (int k = 2; k < KX - 2; ++k) qt[k] = qt[k] + fq[k] * u.rsig[k];
+#pragma unroll
+    for (int k = 2; k < KX - 2; ++k) qt[k + 1] = qt[k + 1] + (-fq[k] * u.rsig[k + 1]);
+    // super-adiabatic damping (:120-131): all local adds, then the cumulative sum
+    float fse[KX];
+#pragma unroll
+    for (int k = 0; k < NL1; ++k) {
+      const float se0 = a.se[k + 1] - p.segrad * (c.PHI[k + 1] - c.PHI[k]);
+      fse[k] = (a.se[k] < se0) ? u.fvdise * (se0 - a.se[k]) : 0.0f;
+      tt[k] = tt[k] + fse[k] * u.rsig[k];
+    }
+    float cum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NL1; ++k) {
+      cum = (k == 0) ? fse[k] * u.rsig1[k] : cum + fse[k] * u.rsig1[k];
+      tt[k + 1] = tt[k + 1] + (-cum);
+    }
+#pragma unroll
+    for (int k = 0; k < KX; ++k) { b.tvd[k] = tt[k]; b.qvd[k] = qt[k]; }
+  }
+}
+
+template <int KX, bool DIAG>
+SPD_HD void phase_b2(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, const SpeedyParams& p,
+                     const SpeedyVertical& v, const Uniforms& u, uint32_t flags, PhaseB<KX>& b) {
+  const float ps = c.ps;
+  const float rps = 1.0f / ps;
   // ---- shortwave (shortwave_radiation.py:27-217)
   float flux_h[4];
-  float tauL[KX][4];
   {
     const float a_cl = (a.icltop < KX) ? p.albcl * a.cloudc : 0.0f;
     float t22[KX];
@@ -457,19 +657,15 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
     const float psaz = ps * c.zenit;
     const float acloud = a.cloudc * fminf(p.abscl1 * a.qcloud, p.abscl2);
     float tau0[KX], tau1[KX];
-    tau0[0] = expf(-psaz * v.dhs[0] * p.absdry);
+    tau0[0] = exp_tr(-psaz * v.dhs[0] * p.absdry);
 #pragma unroll
     for (int k = 1; k < KX - 1; ++k) {
-      const float abs1 = p.absdry + p.absaer * sq(v.fsg[k]);
       const float cl = (k >= a.icltop) ? acloud : 0.0f;
-      tau0[k] = expf(-psaz * v.dhs[k] * (abs1 + p.abswv1 * c.Q[k] + cl));
+      tau0[k] = exp_tr(-psaz * v.dhs[k] * (u.sw_abs1[k] + p.abswv1 * c.Q[k] + cl));
     }
-    {
-      const float abs1 = p.absdry + p.absaer * sq(v.fsg[KX - 1]);
-      tau0[KX - 1] = expf(-psaz * v.dhs[KX - 1] * (abs1 + p.abswv1 * c.Q[KX - 1]));
-    }
+    tau0[KX - 1] = exp_tr(-psaz * v.dhs[KX - 1] * (u.sw_abs1[KX - 1] + p.abswv1 * c.Q[KX - 1]));
 #pragma unroll
-    for (int k = 1; k < KX; ++k) tau1[k] = expf(-psaz * v.dhs[k] * p.abswv2 * c.Q[k]);
+    for (int k = 1; k < KX; ++k) tau1[k] = exp_tr(-psaz * v.dhs[k] * p.abswv2 * c.Q[k]);
     // downward
     float dfabs[KX], f1[KX], f2[KX], R[KX];
     const float F0 = c.fsol * 0.95f;
@@ -496,7 +692,7 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
     for (int k = 1; k < KX; ++k) dfabs[k] = dfabs[k] + f2[k - 1] * (1.0f - tau1[k]);
     // upward
     const float rsds = f1[KX - 1] + f2[KX - 1];
-    float U = f1[KX - 1] * albsfc;
+    float U = f1[KX - 1] * b.albsfc;
     const float rsns = rsds - U;
     float fu[KX];
     fu[KX - 1] = U;
@@ -505,9 +701,9 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
 #pragma unroll
     for (int k = 0; k < KX; ++k) dfabs[k] = dfabs[k] + fu[k] * (1.0f - tau0[k]);
     const float ftoa = tau0[0] * fu[0] + R[0];
-    const bool csw = (flags >> 16) & 1u;   // compute_shortwave (set_physics_flags)
+    const bool csw = u.compute_sw != 0u;
 #pragma unroll
-    for (int k = 0; k < KX; ++k) b.tsw[k] = csw ? dfabs[k] * v.grdscp[k] / ps : 0.0f;
+    for (int k = 0; k < KX; ++k) b.tsw[k] = csw ? dfabs[k] * v.grdscp[k] * rps : 0.0f;
     flux_h[0] = ftoa; flux_h[1] = f2[KX - 1]; flux_h[2] = 0.0f; flux_h[3] = 0.0f;   // Q2
     b.rsds = rsds;
     if (DIAG) {
@@ -516,91 +712,19 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
       b.rsns = rsns;
       b.ftop_sw = c.fsol - ftoa;
     }
-    // longwave transmissivity (:177-200)
-#pragma unroll
-    for (int k = 0; k < KX; ++k) {
-      float ab[4] = {p.ablwin, u.ablco2, p.ablwv1 * c.Q[k], p.ablwv2 * c.Q[k]};
-      if (k == 0) { ab[2] = 0.0f; ab[3] = 0.0f; }
-      if (k >= 2 && k <= KX - 2) {
-        ab[0] = ab[0] + ((k > a.icltop) ? a.cloudc * p.ablcl1 : a.cloudc * p.ablcl2);
-        const float ac2 = a.cloudc * p.ablcl2;
-        ab[2] = fmaxf(ab[2], ac2);
-        ab[3] = fmaxf(ab[3], ac2);
-      }
-#pragma unroll
-      for (int bnd = 0; bnd < 4; ++bnd) tauL[k][bnd] = expf(-ab[bnd] * ps * v.dhs[k]);
-    }
   }
-  const float eps1 = p.epslw / (v.dhs[0] + v.dhs[1]);
-  const float stratc0 = c.stratz * ps, stratc1 = eps1 * ps;
-  const float omeps = 1.0f - p.epslw;
-  // ---- longwave down (longwave_radiation.py:13-117)
-  float st4a0[KX], st4a1[KX], dfl[KX];
-  float rlds;
-  {
-    float Th[KX];
-#pragma unroll
-    for (int k = 0; k < KX - 1; ++k) Th[k] = c.T[k] + u.wvi1[k] * (c.T[k + 1] - c.T[k]);
-    const float s0 = 0.75f * c.T[0] + 0.25f * Th[0];
-    const float s1 = 0.50f * c.T[1] + 0.25f * (Th[0] + Th[1]);
-    float grad[KX];
-#pragma unroll
-    for (int k = 2; k < KX - 1; ++k) grad[k] = 0.5f * fmaxf(Th[k] - Th[k - 1], 0.0f);
-    grad[KX - 1] = fmaxf(c.T[KX - 1] - Th[KX - 2], 0.0f);
-    st4a0[0] = SBC * pow4(s0); st4a0[1] = SBC * pow4(s1);
-    st4a1[0] = 0.0f; st4a1[1] = 0.0f;
-#pragma unroll
-    for (int k = 2; k < KX; ++k) {
-      const float st3a = SBC * pow3(c.T[k]);
-      st4a0[k] = st3a * c.T[k];
-      st4a1[k] = 4.0f * st3a * grad[k];
-    }
-    float Fd[4];
-    {
-      float fb[4];
-      radset4(c.T[0], omeps, fb);
-#pragma unroll
-      for (int bnd = 0; bnd < 4; ++bnd) {
-        const float em = 1.0f - tauL[0][bnd];
-        Fd[bnd] = bnd < 2 ? em * (fb[bnd] * (st4a0[0] + em * st4a1[0])) : 0.0f;
-      }
-    }
-    float Fprev = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
-    dfl[0] = -Fprev;
-#pragma unroll
-    for (int k = 1; k < KX; ++k) {
-      float fb[4];
-      radset4(c.T[k], omeps, fb);
-#pragma unroll
-      for (int bnd = 0; bnd < 4; ++bnd) {
-        const float em = 1.0f - tauL[k][bnd];
-        Fd[bnd] = tauL[k][bnd] * Fd[bnd] + em * (fb[bnd] * (st4a0[k] + em * st4a1[k]));
-      }
-      const float Fs = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
-      dfl[k] = -(Fs - Fprev);
-      Fprev = Fs;
-    }
-    rlds = ((p.emisfc * Fd[0] + p.emisfc * Fd[1]) + p.emisfc * Fd[2]) + p.emisfc * Fd[3];
-    const float corlw = p.epslw * p.emisfc * st4a0[KX - 1];
-    dfl[KX - 1] = dfl[KX - 1] + (-corlw);
-    rlds = rlds + corlw;
-  }
+  const float stratc0 = c.stratz * ps, stratc1 = u.eps1 * ps;
+  const float rlds = b.rlds;
   // ---- surface fluxes (surface_flux.py:15-300)
   float rlus2, tsfc;
   {
     const SpeedyParams& s = p;
-    const float esbc = s.emisfc * SBC;
-    const float ghum0 = 1.0f - s.fhum0;
     const float u1 = c.u1, v1 = c.v1, Tb = c.T[KX - 1], Qb = c.Q[KX - 1];
     const float u0 = s.fwind0 * u1, v0 = s.fwind0 * v1;
-    const float cdl = s.drag_scale != 1.0f ? s.cdl * s.drag_scale : s.cdl;
-    const float cds = s.drag_scale != 1.0f ? s.cds * s.drag_scale : s.cds;
     float t1_0 = 0.0f, t1_1 = 0.0f, t2_1 = 0.0f, q1_1 = 0.0f;
     float den0 = 0.0f, den2 = 0.0f;
     float ustr0 = 0.0f, vstr0 = 0.0f, shf0 = 0.0f, evap0 = 0.0f, rlus0 = 0.0f, hfl0 = 0.0f, tskin = 0.0f;
     float ustr1 = 0.0f, vstr1 = 0.0f;
-    const float astab = s.lscasym ? 0.5f : 1.0f;
-    const float rdth = s.fstab / s.dtheta;
     const bool land = flags & SPEEDY_FLAG_LFLUXLAND;
     if (land) {
       const float phi0 = c.orog * GRAV;
@@ -610,65 +734,58 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
       t2_1 = Tb + RCP * c.PHI[KX - 1];
       const float t2_0 = t2_1 - RCP * phi0;
       if (Tb > c.T[KX - 2]) {
-        const float gtemp0 = 1.0f - s.ftemp0;
-        t1_0 = s.ftemp0 * t1_0 + gtemp0 * t2_0;
-        t1_1 = s.ftemp0 * t1_1 + gtemp0 * t2_1;
+        t1_0 = s.ftemp0 * t1_0 + u.gtemp0 * t2_0;
+        t1_1 = s.ftemp0 * t1_1 + u.gtemp0 * t2_1;
       } else {
         t1_0 = Tb;
         t1_1 = Tb;
       }
       const float t0 = t1_1 + c.fmask * (t1_0 - t1_1);
       den0 = (P0 * ps / (RGAS * t0)) * sqrtf(u0 * u0 + v0 * v0 + s.vgust * s.vgust);
-      tskin = c.stl + s.ctday * c.sqcoa * b.rsds * (1.0f - alb_l) * ps;
+      tskin = c.stl + s.ctday * c.sqcoa * b.rsds * (1.0f - b.alb_l) * ps;
       const float dd = tskin - t2_0;
-      const float dthl = (tskin > t2_0) ? fminf(s.dtheta, dd) : fmaxf(-s.dtheta, astab * dd);
-      const float den1 = den0 * (1.0f + dthl * rdth);
-      const float rhdrag = 1.0f / (GRAV * s.hdrag);
-      const float forog = 1.0f + rhdrag * (1.0f - expf(-fmaxf(c.phis0, 0.0f) * rhdrag));
-      const float cdldv = cdl * den0 * forog;
+      const float dthl = (tskin > t2_0) ? fminf(s.dtheta, dd) : fmaxf(-s.dtheta, u.astab * dd);
+      const float den1 = den0 * (1.0f + dthl * u.rdth);
+      const float forog = 1.0f + u.rhdrag * (1.0f - exp_tr(-fmaxf(c.phis0, 0.0f) * u.rhdrag));
+      const float cdldv = u.cdl * den0 * forog;
       ustr0 = -cdldv * u1;
       vstr0 = -cdldv * v1;
-      const float chlcp = s.chl * CP;
-      shf0 = chlcp * den1 * (tskin - t1_0);
+      shf0 = u.chlcp * den1 * (tskin - t1_0);
       float q1_0 = Qb;
-      float qs_t1 = 0.0f;
-      if (s.fhum0 > 0.0f) {
-        qs_t1 = qsat_f(t1_0, ps, 1.0f);
-        q1_0 = s.fhum0 * (a.rh[KX - 1] * qs_t1) + ghum0 * Qb;
-      }
+      if (s.fhum0 > 0.0f) q1_0 = s.fhum0 * (a.rh[KX - 1] * qsat_f(t1_0, ps, 1.0f)) + u.ghum0 * Qb;
       const float qsat00 = qsat_f(tskin, ps, 1.0f);
       evap0 = s.chl * den1 * fmaxf(0.0f, c.soilw * qsat00 - q1_0);
       const float tsk3 = pow3(tskin);
-      const float drls = 4.0f * esbc * tsk3;
-      rlus0 = esbc * tsk3 * tskin;
-      hfl0 = b.rsds * (1.0f - alb_l) + rlds - (rlus0 + shf0 + (ALHC * evap0));
+      const float drls = 4.0f * u.esbc * tsk3;
+      rlus0 = u.esbc * tsk3 * tskin;
+      hfl0 = b.rsds * (1.0f - b.alb_l) + rlds - (rlus0 + shf0 + (ALHC * evap0));
       if (s.lskineb) {
-        const float clamb = s.clambda + (snowc * (s.clambsn - s.clambda));
+        const float clamb = s.clambda + (b.snowc * (s.clambsn - s.clambda));
         hfl0 = hfl0 - (clamb * (tskin - c.stl));
         float qs1 = qsat_f(tskin + 1.0f, ps, 1.0f);
         qs1 = (evap0 > 0.0f) ? c.soilw * (qs1 - qsat00) : 0.0f;
         const float dtskin = hfl0 / (clamb + drls + (s.chl * den1 * (CP + (ALHC * qs1))));
         tskin = tskin + dtskin;
-        shf0 = shf0 + chlcp * den1 * dtskin;
+        shf0 = shf0 + u.chlcp * den1 * dtskin;
         evap0 = evap0 + s.chl * den1 * qs1 * dtskin;
         rlus0 = rlus0 + drls * dtskin;
         hfl0 = clamb * (tskin - c.stl);
       }
       const float ds = c.sst - t2_1;
-      const float dths = (c.sst > t2_1) ? fminf(s.dtheta, ds) : fmaxf(-s.dtheta, astab * ds);
-      den2 = den0 * (1.0f + dths * rdth);
+      const float dths = (c.sst > t2_1) ? fminf(s.dtheta, ds) : fmaxf(-s.dtheta, u.astab * ds);
+      den2 = den0 * (1.0f + dths * u.rdth);
       q1_1 = Qb;
-      if (s.fhum0 > 0.0f) q1_1 = s.fhum0 * (a.rh[KX - 1] * qsat_f(t1_1, ps, 1.0f)) + ghum0 * Qb;
-      const float cdsdv = cds * den2;
+      if (s.fhum0 > 0.0f) q1_1 = s.fhum0 * (a.rh[KX - 1] * qsat_f(t1_1, ps, 1.0f)) + u.ghum0 * Qb;
+      const float cdsdv = u.cds * den2;
       ustr1 = -cdsdv * u1;
       vstr1 = -cdsdv * v1;
     }
     // sea (:250-261)
-    const float shf1 = s.chs * CP * den2 * (c.sst - t1_1);
+    const float shf1 = u.chscp * den2 * (c.sst - t1_1);
     const float qsat01 = qsat_f(c.sst, ps, 1.0f);
     const float evap1 = s.chs * den2 * (qsat01 - q1_1);
-    const float rlus1 = esbc * pow4(c.sst);
-    const float hfl1 = b.rsds * (1.0f - alb_s) + rlds - rlus1 + shf1 + ALHC * evap1;
+    const float rlus1 = u.esbc * pow4(c.sst);
+    const float hfl1 = b.rsds * (1.0f - b.alb_s) + rlds - rlus1 + shf1 + ALHC * evap1;
     float ustr2 = 0.0f, vstr2 = 0.0f, shf2 = 0.0f, evap2 = 0.0f, t0o = 0.0f;
     rlus2 = 0.0f; tsfc = 0.0f;
     float tskin_o = 0.0f;
@@ -683,7 +800,6 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
       tsfc = c.sst + fm * (c.stl - c.sst);
       tskin_o = c.sst + fm * (tskin - c.sst);
     }
-    const float rps = 1.0f / ps;
     b.du1 = ustr2 * rps * v.grdsig[KX - 1];
     b.dv1 = vstr2 * rps * v.grdsig[KX - 1];
     b.dts1 = shf2 * rps * v.grdscp[KX - 1];
@@ -701,21 +817,23 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
   // ---- longwave up (longwave_radiation.py:119-202)
   {
     float fbt[4];
-    radset4(tsfc, omeps, fbt);
-    const float refsfc = 1.0f - p.emisfc;
+    radset4(tsfc, u.omeps, fbt);
     float Fu[4];
 #pragma unroll
-    for (int bnd = 0; bnd < 4; ++bnd) Fu[bnd] = fbt[bnd] * rlus2 + refsfc * flux_h[bnd];
+    for (int bnd = 0; bnd < 4; ++bnd) Fu[bnd] = fbt[bnd] * rlus2 + u.refsfc * flux_h[bnd];
+    float dfl[KX];
+#pragma unroll
+    for (int k = 0; k < KX; ++k) dfl[k] = b.dfl[k];
     dfl[KX - 1] = dfl[KX - 1] + p.epslw * rlus2;
     float Gnext = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
 #pragma unroll
     for (int k = KX - 1; k >= 1; --k) {
       float fb[4];
-      radset4(c.T[k], omeps, fb);
+      radset4(c.T[k], u.omeps, fb);
 #pragma unroll
       for (int bnd = 0; bnd < 4; ++bnd) {
-        const float em = 1.0f - tauL[k][bnd];
-        Fu[bnd] = tauL[k][bnd] * Fu[bnd] + em * (fb[bnd] * (st4a0[k] - em * st4a1[k]));
+        const float em = 1.0f - b.tauL[k][bnd];
+        Fu[bnd] = b.tauL[k][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[k] - em * b.st4a1[k]));
       }
       const float Gk = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
       dfl[k] = dfl[k] + (Gnext - Gk);
@@ -724,101 +842,31 @@ SPD_HD void phase_b(const ColumnIn<KX>& c, const PhaseA<KX>& a, uint32_t S, cons
     float ft[4];
     {
       float fb[4];
-      radset4(c.T[0], omeps, fb);
+      radset4(c.T[0], u.omeps, fb);
 #pragma unroll
       for (int bnd = 0; bnd < 2; ++bnd) {
-        const float em = 1.0f - tauL[0][bnd];
-        ft[bnd] = tauL[0][bnd] * Fu[bnd] + em * (fb[bnd] * (st4a0[0] - em * st4a1[0]));
+        const float em = 1.0f - b.tauL[0][bnd];
+        ft[bnd] = b.tauL[0][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[0] - em * b.st4a1[0]));
       }
       ft[2] = Fu[2];
       ft[3] = Fu[3];
     }
     dfl[0] = dfl[0] + ((Fu[0] - ft[0]) + (Fu[1] - ft[1]));
-    const float corlw1 = v.dhs[0] * stratc1 * st4a0[0] + stratc0;
-    const float corlw2 = v.dhs[1] * stratc1 * st4a0[1];
+    const float corlw1 = v.dhs[0] * stratc1 * b.st4a0[0] + stratc0;
+    const float corlw2 = v.dhs[1] * stratc1 * b.st4a0[1];
     dfl[0] = dfl[0] + (-corlw1);
     dfl[1] = dfl[1] + (-corlw2);
 #pragma unroll
-    for (int k = 0; k < KX; ++k) b.tlw[k] = dfl[k] * v.grdscp[k] / ps;
+    for (int k = 0; k < KX; ++k) b.tlw[k] = dfl[k] * v.grdscp[k] * rps;
     if (DIAG) {
       b.ftop_lw = (corlw1 + corlw2) + (((ft[0] + ft[1]) + ft[2]) + ft[3]);
 #pragma unroll
       for (int bnd = 0; bnd < 4; ++bnd) b.flux_out[bnd] = ft[bnd];
-      b.rlds = rlds;
       b.rlns = rlus2 - rlds;
       b.stratc0 = stratc0; b.stratc1 = stratc1;
 #pragma unroll
-      for (int k = 0; k < KX; ++k) {
-        b.dfabs_lw[k] = dfl[k];
-        b.st4a0[k] = st4a0[k];
-        b.st4a1[k] = st4a1[k];
-#pragma unroll
-        for (int bnd = 0; bnd < 4; ++bnd) b.tau[k][bnd] = tauL[k][bnd];
-      }
+      for (int k = 0; k < KX; ++k) b.dfl[k] = dfl[k];
     }
-  }
-  // ---- vertical diffusion (vertical_diffusion.py:10-135)
-  {
-    const int icnv = KX - a.iptop - 1;
-    constexpr int NL1 = KX - 1;
-    const float cshc = v.dhs[KX - 1] / 3600.0f;
-    const float cvdi = (v.hsg[NL1] - v.hsg[1]) / (float)((NL1 - 1) * 3600.0);
-    const float fshcq = cshc / p.trshc;
-    const float fshcse = cshc / (p.trshc * CP);
-    const float fvdiq = cvdi / p.trvdi;
-    const float fvdise = cvdi / (p.trvds * CP);
-    float tt[KX], qt[KX];
-#pragma unroll
-    for (int k = 0; k < KX; ++k) { tt[k] = 0.0f; qt[k] = 0.0f; }
-    const float drh0 = p.rhgrad * (v.fsg[KX - 1] - v.fsg[NL1 - 1]);
-    const float fvdiq2 = fvdiq * v.hsg[NL1];
-    const float dmse = a.se[KX - 1] - a.se[NL1 - 1] + ALHC * (c.Q[KX - 1] - a.qsat[NL1 - 1]);
-    const float drh = a.rh[KX - 1] - a.rh[NL1 - 1];
-    const float fcnv = (icnv > 0 && dmse >= 0.0f) ? p.redshc : 1.0f;
-    const float rsA = 1.0f / v.dhs[NL1 - 1], rsB = 1.0f / v.dhs[KX - 1];
-    if (dmse >= 0.0f) {
-      const float fluxse = fcnv * fshcse * dmse;
-      tt[NL1 - 1] = fluxse * rsA;
-      tt[KX - 1] = -fluxse * rsB;
-      if (drh >= 0.0f) {
-        const float fluxq = fcnv * fshcq * a.qsat[KX - 1] * drh;
-        qt[NL1 - 1] = fluxq * rsA;
-        qt[KX - 1] = -fluxq * rsB;
-      }
-    } else if (dmse < 0.0f && drh > drh0) {
-      const float fluxq = fvdiq2 * a.qsat[NL1 - 1] * drh;
-      qt[NL1 - 1] = fluxq * rsA;
-      qt[KX - 1] = -fluxq * rsB;
-    }
-    // moisture diffusion above the PBL (:99-118)
-    float fq[KX];
-#pragma unroll
-    for (int k = 2; k < KX - 2; ++k) {
-      const float d = a.rh[k + 1] - a.rh[k];
-      const float d0 = p.rhgrad * (v.fsg[k + 1] - v.fsg[k]);
-      fq[k] = (d >= d0 && v.hsg[k + 1] > 0.5f) ? fvdiq * v.hsg[k + 1] * a.qsat[k] * d : 0.0f;
-    }
-#pragma unroll
-    for (int k = 2; k < KX - 2; ++k) qt[k] = qt[k] + fq[k] * (1.0f / v.dhs[k]);
-#pragma unroll
-    for (int k = 2; k < KX - 2; ++k) qt[k + 1] = qt[k + 1] + (-fq[k] * (1.0f / v.dhs[k + 1]));
-    // super-adiabatic damping (:120-131): all local adds, then the cumulative sum
-    float fse[KX];
-#pragma unroll
-    for (int k = 0; k < NL1; ++k) {
-      const float se0 = a.se[k + 1] - p.segrad * (c.PHI[k + 1] - c.PHI[k]);
-      fse[k] = (a.se[k] < se0) ? fvdise * (se0 - a.se[k]) : 0.0f;
-      tt[k] = tt[k] + fse[k] * (1.0f / v.dhs[k]);
-    }
-    float cum = 0.0f;
-#pragma unroll
-    for (int k = 0; k < NL1; ++k) {
-      const float r1 = 1.0f / (1.0f - v.hsg[k + 1]);
-      cum = (k == 0) ? fse[k] * r1 : cum + fse[k] * r1;
-      tt[k + 1] = tt[k + 1] + (-cum);
-    }
-#pragma unroll
-    for (int k = 0; k < KX; ++k) { b.tvd[k] = tt[k]; b.qvd[k] = qt[k]; }
   }
 }
 

@@ -1,21 +1,33 @@
 // speedy_kernels.hip -- gfx950 kernels and the C ABI (include/speedy_abi.h) of the
 // SPEEDY column physics.
 //
-// One call of the physics is two launches on the caller's stream:
-//   1. cloudtop_kernel: per-latitude insolation/ozone tables, and the Q1 level
-//      set S of every member (an OR over all of the member's columns of
-//      1 << clip(icltop, 0, kx-1); shortwave_radiation.py:63-70).  The last
-//      block to finish publishes S and clears the accumulators, so no memset
-//      is needed between calls.
-//   2. fwd_kernel: one thread per column, the whole column held in registers;
-//      phase A (humidity, convection, condensation, clouds) is recomputed
-//      rather than stored, phase B (shortwave with S, longwave, surface fluxes,
-//      vertical diffusion) follows, the term-ordered tendency sum is written.
-// Reads/writes are [field][level][column] with consecutive threads on
-// consecutive columns, so every access is a coalesced 256-byte wave load.
+// The eleven terms are column-local except for one grid-wide coupling: the
+// shortwave cloud albedo is applied at every level of the set
+//   S = { clip(icltop[c], 0, kx-1) : every column c of the call }
+// (shortwave_radiation.py:63-70, an advanced-index scatter).  S is an OR of
+// one bit per column, so a call needs a grid-wide reduction between the cloud
+// top (end of phase A) and the shortwave (phase B).  Two paths:
+//
+//   fused (the grid fits on the GPU at once, e.g. T85 = 128 blocks):
+//     one cooperative launch.  Right after the cloud top every block publishes
+//     its columns' OR of level bits as one tagged word in its own slot (tag =
+//     the call's generation), computes the S-independent work (longwave
+//     transmissivity, longwave down, vertical diffusion), then polls only the
+//     slots of the blocks covering its member until every tag is current.
+//     Block 0 then advances the generation.  No read-modify-write atomics:
+//     contended same-address atomics cost microseconds per call on this GPU.
+//   two-kernel (larger grids): levelset_kernel, a fixed number of blocks each
+//     covering a contiguous chunk of columns, writes per-(chunk, member) ORs;
+//     fwd_kernel ORs the few chunks of its member, recomputes phase A and does
+//     phase B.
+//
+// One thread per column; the column lives in registers.  Accesses are
+// [field][level][column] with consecutive lanes on consecutive columns
+// (coalesced 256-byte wave loads/stores).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -26,17 +38,24 @@
 namespace speedy {
 
 constexpr int kBlock = 256;
+constexpr uint32_t kSpinLimit = 4u << 20;   // barrier wait bound (~seconds): a wave always exits
+
+struct Divisor {          // n / d for n < 2^32 by multiply-high + one correction step
+  uint32_t d, m;          // m = floor(2^32 / d) (unused when d == 1)
+};
 
 struct KArgs {
   SpeedyFields f;
   SpeedyVertical v;
   SpeedyParams p;
-  SpeedyDate d;
-  int32_t nlon, nlat, nmember, level_set;
-  int64_t ncol, ntot;
+  Uniforms u;
+  uint32_t ncol, ntot, nmember, nlat;
+  uint32_t nblk;                     // blocks of kBlock columns
+  uint32_t ls_blocks, chunk, jmax;   // two-kernel level-set pass: chunks of `chunk` columns
+  Divisor dcol, dlat, dchunk;
+  int32_t level_set;
   uint32_t flags, term_mask;
-  uint32_t *counter, *s_accum, *s_final;
-  float *lat;   // [6][nlat]
+  uint32_t *gen, *err, *last_s, *partial;
 };
 
 // ---------------------------------------------------------------- diagnostics
@@ -66,11 +85,18 @@ constexpr int kN3D = sizeof(kDiag3D) / sizeof(kDiag3D[0]);
 
 constexpr int diag_rows(int kx) { return kN2D + kN3D * kx; }
 
-// ---------------------------------------------------------------- loads
+// ---------------------------------------------------------------- device helpers
+__device__ __forceinline__ uint32_t divq(uint32_t n, const Divisor& dv) {
+  if (dv.d == 1u) return n;
+  uint32_t q = __umulhi(n, dv.m);
+  if (n - q * dv.d >= dv.d) q += 1u;
+  return q;
+}
+
 template <int KX>
-__device__ __forceinline__ void load_column(const KArgs& a, int64_t m, int64_t c, ColumnIn<KX>& col) {
-  const int64_t ncol = a.ncol;
-  const int64_t base3 = m * KX * ncol + c;
+__device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t c, ColumnIn<KX>& col) {
+  const size_t ncol = a.ncol;
+  const size_t base3 = (size_t)m * KX * ncol + c;
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
     col.T[k] = a.f.t[base3 + k * ncol];
@@ -79,7 +105,7 @@ __device__ __forceinline__ void load_column(const KArgs& a, int64_t m, int64_t c
   }
   col.u1 = a.f.u[base3 + (KX - 1) * ncol];
   col.v1 = a.f.v[base3 + (KX - 1) * ncol];
-  col.ps = a.f.ps[m * ncol + c];
+  col.ps = a.f.ps[(size_t)m * ncol + c];
   col.alb0 = a.f.alb0[c];
   col.sice = a.f.sice_am[c];
   col.snowc_am = a.f.snowc_am[c];
@@ -89,90 +115,23 @@ __device__ __forceinline__ void load_column(const KArgs& a, int64_t m, int64_t c
   col.fmask = a.f.fmask[c];
   col.orog = a.f.orog[c];
   col.phis0 = a.f.phis0[c];
-  const int64_t j = c % a.nlat;
-  const int64_t nl = a.nlat;
-  col.fsol = a.lat[0 * nl + j];
-  col.ozupp = a.lat[1 * nl + j];
-  col.ozone = a.lat[2 * nl + j];
-  col.zenit = a.lat[3 * nl + j];
-  col.stratz = a.lat[4 * nl + j];
-  col.sqcoa = a.lat[5 * nl + j];
 }
 
-// ---------------------------------------------------------------- kernels
 template <int KX>
-__global__ __launch_bounds__(kBlock) void cloudtop_kernel(KArgs a) {
-  __shared__ uint32_t s_or[kBlock + 1];
-  __shared__ int s_last;
-  const int tid = threadIdx.x;
-  const int64_t gid = (int64_t)blockIdx.x * kBlock + tid;
-  if (gid < a.nlat) {
-    float z[6];
-    zonal_fields(a.d.tyear, a.f.sia[gid], a.f.coa[gid], z);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) a.lat[i * (int64_t)a.nlat + gid] = z[i];
-  }
-  if (a.level_set >= 0) return;   // uniform over the grid: S is given
-  const int64_t blk0 = (int64_t)blockIdx.x * kBlock;
-  const int64_t blk_last = min(blk0 + kBlock, a.ntot) - 1;
-  const int64_t m_first = blk0 / a.ncol;
-  const int nm = (int)(blk_last / a.ncol - m_first + 1);
-  for (int i = tid; i < nm; i += kBlock) s_or[i] = 0u;
-  __syncthreads();
-  if (gid < a.ntot) {
-    const int64_t m = gid / a.ncol;
-    const int64_t c = gid - m * a.ncol;
-    ColumnIn<KX> col;
-    const int64_t ncol = a.ncol;
-    const int64_t base3 = m * KX * ncol + c;
-#pragma unroll
-    for (int k = 1; k < KX; ++k) {
-      col.T[k] = a.f.t[base3 + k * ncol];
-      col.Q[k] = a.f.q[base3 + k * ncol];
-      col.PHI[k] = a.f.phi[base3 + k * ncol];
-    }
-    col.ps = a.f.ps[m * ncol + c];
-    Uniforms<KX> u;
-    make_uniforms<KX>(a.p, a.v, a.d, u);
-    const int icl = cloud_top<KX>(col, a.p, a.v, u);
-    const int lv = icl < 0 ? 0 : (icl > KX - 1 ? KX - 1 : icl);
-    atomicOr(&s_or[m - m_first], 1u << lv);
-  }
-  __syncthreads();
-  for (int i = tid; i < nm; i += kBlock)
-    if (s_or[i]) atomicOr(&a.s_accum[m_first + i], s_or[i]);
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    const unsigned prev = atomicInc(a.counter, gridDim.x - 1);   // wraps to 0 for the next call
-    s_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (s_last) {
-    __threadfence();
-    for (int m = tid; m < a.nmember; m += kBlock) a.s_final[m] = atomicExch(&a.s_accum[m], 0u);
-  }
+__device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<KX>& col) {
+  const uint32_t j = c - divq(c, a.dlat) * a.nlat;
+  float z[6];
+  zonal_column(a.u.zon, a.f.sia[j], a.f.coa[j], z);
+  col.fsol = z[0]; col.ozupp = z[1]; col.ozone = z[2]; col.zenit = z[3]; col.stratz = z[4]; col.sqcoa = z[5];
 }
 
 template <int KX, bool DIAG>
-__global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
-  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (gid >= a.ntot) return;
-  const int64_t m = gid / a.ncol;
-  const int64_t c = gid - m * a.ncol;
-  const int64_t ncol = a.ncol;
-  ColumnIn<KX> col;
-  load_column<KX>(a, m, c, col);
-  Uniforms<KX> u;
-  make_uniforms<KX>(a.p, a.v, a.d, u);
-  PhaseA<KX> pa;
-  phase_a<KX>(col, a.p, a.v, u, pa);
-  const uint32_t S = a.level_set >= 0 ? (uint32_t)a.level_set : a.s_final[m];
-  PhaseB<KX> pb;
-  phase_b<KX, DIAG>(col, pa, S, a.p, a.v, u, a.flags, pb);
+__device__ __forceinline__ void store_outputs(const KArgs& a, uint32_t m, uint32_t c, const ColumnIn<KX>& col,
+                                              const PhaseA<KX>& pa, const PhaseB<KX>& pb) {
   // term-ordered sum (speedy_physics.py:123-125)
   const uint32_t tm = a.term_mask;
-  const int64_t base3 = m * KX * ncol + c;
+  const size_t ncol = a.ncol;
+  const size_t base3 = (size_t)m * KX * ncol + c;
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
     float t = 0.0f, q = 0.0f, du = 0.0f, dv = 0.0f;
@@ -190,9 +149,9 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
     a.f.dq[base3 + k * ncol] = q;
   }
   if (DIAG) {
-    float* dg = a.f.diag + m * (int64_t)diag_rows(KX) * ncol + c;
+    float* dg = a.f.diag + (size_t)m * diag_rows(KX) * ncol + c;
     int r = 0;
-    auto W = [&](float x) { dg[(int64_t)(r++) * ncol] = x; };
+    auto W = [&](float x) { dg[(size_t)(r++) * ncol] = x; };
     W((float)pa.iptop); W(pa.cbmf); W(pa.precnv); W(pa.precls);
     W((float)pa.icltop); W(pa.cloudc); W(pa.clstr); W(pa.qcloud); W(pa.gse);
     W(pb.rsds); W(pb.rsns); W(pb.ftop_sw);
@@ -216,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
 #pragma unroll
     for (int k = 0; k < KX; ++k) W(pb.dfabs_sw[k]);
 #pragma unroll
-    for (int k = 0; k < KX; ++k) W(pb.dfabs_lw[k]);
+    for (int k = 0; k < KX; ++k) W(pb.dfl[k]);
 #pragma unroll
     for (int k = 0; k < KX; ++k) W(pb.st4a0[k]);
 #pragma unroll
@@ -224,8 +183,153 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
-      for (int k = 0; k < KX; ++k) W(pb.tau[k][b]);
+      for (int k = 0; k < KX; ++k) W(pb.tauL[k][b]);
   }
+}
+
+// ---------------------------------------------------------------- level set S
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t gen_tag(uint32_t g) { return (g & 0x7fffu) | 0x8000u; }
+
+__device__ __forceinline__ uint32_t clip_bit(int icl, int kx) {
+  return 1u << (icl < 0 ? 0 : (icl > kx - 1 ? kx - 1 : icl));
+}
+
+// Fused path: OR of the level bits of member m over the published words of the
+// blocks covering it, polled until every word carries `tag` (one wave).
+// Bounded: after kSpinLimit polls it flags the error word and returns.
+__device__ uint32_t collect_fused(const KArgs& a, uint32_t tag, uint32_t m) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t b0 = (m * a.ncol) / kBlock, b1 = (m * a.ncol + a.ncol - 1u) / kBlock;
+  for (uint32_t spins = 0;; ++spins) {
+    uint32_t acc = 0;
+    int stale = 0;
+    for (uint32_t b = b0 + lane; b <= b1; b += 64u) {
+      const uint32_t j = m - divq(b * kBlock, a.dcol);
+      const uint32_t w = __hip_atomic_load(&a.partial[2u * b + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      stale |= (w >> 16) != tag;
+      acc |= w & 0xffffu;
+    }
+    if (!__any(stale)) return wave_or(acc);
+    if (spins >= kSpinLimit) {
+      if (lane == 0) __hip_atomic_fetch_or(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return wave_or(acc);
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Two-kernel path: OR over the chunks of levelset_kernel that cover member m.
+__device__ uint32_t collect_chunks(const KArgs& a, uint32_t m) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
+  uint32_t acc = 0;
+  for (uint32_t b = b0 + lane; b <= b1; b += 64u) acc |= a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
+  return wave_or(acc);
+}
+
+// ---------------------------------------------------------------- kernels
+// Two-kernel path, launch 1: every column's cloud top, ORed per (chunk, member).
+template <int KX>
+__global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
+  extern __shared__ uint32_t s_or[];   // jmax words
+  const uint32_t c_begin = blockIdx.x * a.chunk;
+  const uint32_t c_end = min(c_begin + a.chunk, a.ntot);
+  const uint32_t m_first = divq(c_begin, a.dcol);
+  const uint32_t nm = divq(c_end - 1u, a.dcol) - m_first + 1u;
+  for (uint32_t i = threadIdx.x; i < nm; i += kBlock) s_or[i] = 0u;
+  __syncthreads();
+  for (uint32_t gid = c_begin + threadIdx.x; gid < c_end; gid += kBlock) {
+    const uint32_t m = divq(gid, a.dcol);
+    const uint32_t c = gid - m * a.ncol;
+    ColumnIn<KX> col;
+    const size_t ncol = a.ncol;
+    const size_t base3 = (size_t)m * KX * ncol + c;
+#pragma unroll
+    for (int k = 1; k < KX; ++k) {
+      col.T[k] = a.f.t[base3 + k * ncol];
+      col.Q[k] = a.f.q[base3 + k * ncol];
+      col.PHI[k] = a.f.phi[base3 + k * ncol];
+    }
+    col.ps = a.f.ps[(size_t)m * ncol + c];
+    atomicOr(&s_or[m - m_first], clip_bit(cloud_top<KX>(col, a.p, a.v, a.u), KX));
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[blockIdx.x * a.jmax + i] = s_or[i];
+}
+
+// Forward.  FUSED: single launch with the split grid barrier (all blocks
+// co-resident); otherwise S comes from levelset_kernel of the same call.
+template <int KX, bool DIAG, bool FUSED>
+__global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
+  __shared__ uint32_t s_or[2], s_s[2], s_tag;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = gid < a.ntot;
+  const uint32_t g0 = gid < a.ntot ? gid : a.ntot - 1u;   // inactive lanes shadow a valid column
+  const uint32_t m = divq(g0, a.dcol);
+  const uint32_t c = g0 - m * a.ncol;
+  const uint32_t blk0 = blockIdx.x * kBlock;
+  const uint32_t m_first = divq(blk0, a.dcol);
+  const uint32_t nm = divq(min(blk0 + kBlock, a.ntot) - 1u, a.dcol) - m_first + 1u;   // <= 2 when fused
+  ColumnIn<KX> col;
+  load_state<KX>(a, m, c, col);
+  PhaseA<KX> pa;
+  phase_a<KX>(col, a.p, a.v, a.u, pa);
+  const bool need_s = a.level_set < 0;
+  if (FUSED && need_s) {
+    // publish this block's word(s) for the call's level set
+    if (threadIdx.x == 0) {
+      s_or[0] = 0u; s_or[1] = 0u;
+      s_tag = gen_tag(__hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    __syncthreads();
+    if (active) atomicOr(&s_or[m - m_first], clip_bit(pa.icltop, KX));
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (uint32_t j = 0; j < nm; ++j)
+        __hip_atomic_store(&a.partial[2u * blockIdx.x + j], (s_tag << 16) | s_or[j], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+  PhaseB<KX> pb;
+  phase_b1<KX>(col, pa, a.p, a.v, a.u, pb);
+  uint32_t S = (uint32_t)a.level_set;
+  const bool wide = a.ncol >= (uint32_t)kBlock;   // a block spans at most two members
+  if (need_s) {
+    if (threadIdx.x < 64u) {   // wave 0 gathers S for the block's member(s)
+      for (uint32_t j = 0; wide && j < nm; ++j) {
+        const uint32_t v = FUSED ? collect_fused(a, s_tag, m_first + j) : collect_chunks(a, m_first + j);
+        if (threadIdx.x == 0) s_s[j] = v;
+      }
+      if (blockIdx.x == 0) {
+        // every member's S for speedy_read_level_sets; in the fused path this also
+        // waits for every block's word, i.e. every block has read the generation
+        for (uint32_t mm = 0; mm < a.nmember; ++mm) {
+          const uint32_t v = FUSED ? collect_fused(a, s_tag, mm) : collect_chunks(a, mm);
+          if (threadIdx.x == 0) a.last_s[mm] = v;
+        }
+        if (FUSED && threadIdx.x == 0) {
+          const uint32_t g = __hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(a.gen, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    __syncthreads();
+    if (wide) {
+      S = s_s[m - m_first];
+    } else {   // two-kernel path, members narrower than a block: each spans <= 2 chunks
+      S = 0u;
+      const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
+      for (uint32_t b = b0; b <= b1; ++b) S |= a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
+    }
+  }
+  load_zonal<KX>(a, c, col);
+  phase_b2<KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
+  if (active) store_outputs<KX, DIAG>(a, m, c, col, pa, pb);
 }
 
 __global__ void fill_zero_kernel(uint32_t* p, int64_t n) {
@@ -241,7 +345,7 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-// ---------------------------------------------------------------- kernel timers
+// kernel timers
 struct Timer {
   std::mutex mu;
   bool on = false;
@@ -267,18 +371,45 @@ void timer_end(hipStream_t s, int slot) {
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-struct WsLayout {
-  size_t counter, s_accum, s_final, lat, total;
+// Workspace: word [0] generation, [1] error word; from byte 256 last_s[nmember];
+// then the published level-set words (max of the fused path's two per block
+// and the two-kernel path's jmax per chunk).
+struct Grid {
+  uint32_t ncol, ntot, nblk, ls_blocks, chunk, jmax;
 };
 
-WsLayout ws_layout(int32_t nlat, int32_t nmember) {
+Grid make_grid(int64_t nlon, int64_t nlat, int64_t nmember) {
+  Grid g;
+  g.ncol = (uint32_t)(nlon * nlat);
+  g.ntot = (uint32_t)(nlon * nlat * nmember);
+  g.nblk = (g.ntot + kBlock - 1) / kBlock;
+  const uint32_t lsb = g.nblk < 1024u ? g.nblk : 1024u;   // ~4 blocks per CU: latency hiding
+  g.chunk = ((g.ntot + lsb - 1) / lsb + kBlock - 1) / kBlock * kBlock;
+  g.ls_blocks = (g.ntot + g.chunk - 1) / g.chunk;
+  g.jmax = (g.chunk - 1) / g.ncol + 2;
+  return g;
+}
+
+struct WsLayout {
+  size_t last_s, partial, total;
+};
+
+WsLayout ws_layout(const Grid& g, int32_t nmember) {
   WsLayout L;
-  L.counter = 0;
-  L.s_accum = 256;
-  L.s_final = L.s_accum + align256(sizeof(uint32_t) * (size_t)nmember);
-  L.lat = L.s_final + align256(sizeof(uint32_t) * (size_t)nmember);
-  L.total = L.lat + align256(sizeof(float) * 6 * (size_t)nlat);
+  L.last_s = 256;
+  L.partial = L.last_s + align256(sizeof(uint32_t) * (size_t)nmember);
+  const size_t npart = std::max<size_t>(2 * (size_t)g.nblk, (size_t)g.ls_blocks * g.jmax);
+  L.total = L.partial + align256(sizeof(uint32_t) * npart);
   return L;
+}
+
+constexpr uint32_t kMaxJ = 16384;   // LDS words of levelset_kernel's per-member ORs
+
+Divisor make_divisor(uint32_t d) {
+  Divisor dv;
+  dv.d = d;
+  dv.m = d > 1 ? (uint32_t)(((uint64_t)1 << 32) / d) : 0u;
+  return dv;
 }
 
 int check_fields(const SpeedyFields* f) {
@@ -303,8 +434,10 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   if (int rc = check_fields(fields)) return rc;
   const int64_t ncol = (int64_t)nlon * nlat;
   const int64_t ntot = ncol * nmember;
-  if (ntot > ((int64_t)1 << 40)) return fail(SPEEDY_ERR_CAPACITY, "speedy: too many columns");
-  const WsLayout L = ws_layout(nlat, nmember);
+  if (ntot >= ((int64_t)1 << 31) - kBlock) return fail(SPEEDY_ERR_CAPACITY, "speedy: more than 2^31 columns per call");
+  const Grid gr = make_grid(nlon, nlat, nmember);
+  if (gr.jmax > kMaxJ) return fail(SPEEDY_ERR_CAPACITY, "speedy: too many members per column chunk");
+  const WsLayout L = ws_layout(gr, nmember);
   if (!workspace) return fail(SPEEDY_ERR_WORKSPACE, "speedy: workspace is NULL");
   if (workspace_bytes < L.total)
     return fail(SPEEDY_ERR_WORKSPACE, "speedy: workspace too small (need " + std::to_string(L.total) + " bytes)");
@@ -312,38 +445,73 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   a.f = *fields;
   a.v = *vert;
   a.p = *params;
-  a.d = *date;
-  a.nlon = nlon; a.nlat = nlat; a.nmember = nmember; a.level_set = level_set;
-  a.ncol = ncol; a.ntot = ntot;
-  // compute_shortwave = model_step % nstrad == 0 (speedy_physics.py:26-27, nstrad=3)
-  int32_t ms = date->model_step % 3;
-  if (ms < 0) ms += 3;
-  a.flags = flags | ((ms == 0) ? (1u << 16) : 0u);
+  make_uniforms(*params, *vert, *date, a.u);
+  a.ncol = gr.ncol; a.ntot = gr.ntot; a.nmember = (uint32_t)nmember; a.nlat = (uint32_t)nlat;
+  a.nblk = gr.nblk; a.ls_blocks = gr.ls_blocks; a.chunk = gr.chunk; a.jmax = gr.jmax;
+  a.dcol = make_divisor(gr.ncol);
+  a.dlat = make_divisor((uint32_t)nlat);
+  a.dchunk = make_divisor(gr.chunk);
+  a.level_set = level_set;
+  a.flags = flags;
   a.term_mask = term_mask;
   char* w = static_cast<char*>(workspace);
-  a.counter = reinterpret_cast<uint32_t*>(w + L.counter);
-  a.s_accum = reinterpret_cast<uint32_t*>(w + L.s_accum);
-  a.s_final = reinterpret_cast<uint32_t*>(w + L.s_final);
-  a.lat = reinterpret_cast<float*>(w + L.lat);
+  a.gen = reinterpret_cast<uint32_t*>(w);
+  a.err = a.gen + 1;
+  a.last_s = reinterpret_cast<uint32_t*>(w + L.last_s);
+  a.partial = reinterpret_cast<uint32_t*>(w + L.partial);
+  return SPEEDY_OK;
+}
+
+// Largest grid that is co-resident on the current device for a kernel.
+template <typename K>
+int64_t coresident_blocks(K kernel) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess) return 0;
+  return (int64_t)per_cu * ncu;
+}
+
+template <int KX, bool DIAG>
+int launch_fwd(const KArgs& a, hipStream_t s) {
+  static int64_t cap = -1;   // one GPU per process in this library's deployments
+  if (cap < 0) cap = coresident_blocks(fwd_kernel<KX, DIAG, true>);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(s, &cs);
+  const bool fused = a.level_set < 0 && (int64_t)a.nblk <= cap && a.ncol >= (uint32_t)kBlock;
+  hipError_t e = hipSuccess;
+  if (fused) {
+    const int slot = timer_begin(s, 1);
+    if (cs == hipStreamCaptureStatusNone) {
+      void* args[] = {const_cast<KArgs*>(&a)};
+      e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&fwd_kernel<KX, DIAG, true>), dim3(a.nblk),
+                                     dim3(kBlock), args, 0, s);
+    } else {
+      // Graph capture: a plain launch of a grid that fits on the device; the
+      // level-set poll is bounded, so the kernel terminates in every case.
+      hipLaunchKernelGGL((fwd_kernel<KX, DIAG, true>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+      e = hipGetLastError();
+    }
+    timer_end(s, slot);
+  } else {
+    int slot = -1;
+    if (a.level_set < 0) {
+      slot = timer_begin(s, 0);
+      hipLaunchKernelGGL(levelset_kernel<KX>, dim3(a.ls_blocks), dim3(kBlock), a.jmax * sizeof(uint32_t), s, a);
+      timer_end(s, slot);
+    }
+    slot = timer_begin(s, 1);
+    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+    timer_end(s, slot);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: launch failed: ") + hipGetErrorString(e));
   return SPEEDY_OK;
 }
 
 template <int KX>
-int launch_fwd(const KArgs& a, hipStream_t s) {
-  const int64_t nblk = (a.ntot + kBlock - 1) / kBlock;
-  if (nblk > 0x7fffffff) return fail(SPEEDY_ERR_CAPACITY, "speedy: grid too large");
-  int slot = timer_begin(s, 0);
-  hipLaunchKernelGGL(cloudtop_kernel<KX>, dim3((unsigned)nblk), dim3(kBlock), 0, s, a);
-  timer_end(s, slot);
-  slot = timer_begin(s, 1);
-  if (a.f.diag)
-    hipLaunchKernelGGL((fwd_kernel<KX, true>), dim3((unsigned)nblk), dim3(kBlock), 0, s, a);
-  else
-    hipLaunchKernelGGL((fwd_kernel<KX, false>), dim3((unsigned)nblk), dim3(kBlock), 0, s, a);
-  timer_end(s, slot);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: launch failed: ") + hipGetErrorString(e));
-  return SPEEDY_OK;
+int launch_fwd_kx(const KArgs& a, hipStream_t s) {
+  return a.f.diag ? launch_fwd<KX, true>(a, s) : launch_fwd<KX, false>(a, s);
 }
 
 }  // namespace speedy
@@ -354,9 +522,10 @@ extern "C" {
 
 int speedy_abi_version(void) { return SPEEDY_ABI_VERSION; }
 
-size_t speedy_workspace_bytes(int32_t nlat, int32_t nmember) {
-  if (nlat <= 0 || nmember <= 0) return 0;
-  return ws_layout(nlat, nmember).total;
+size_t speedy_workspace_bytes(int32_t nlon, int32_t nlat, int32_t nmember) {
+  if (nlon <= 0 || nlat <= 0 || nmember <= 0) return 0;
+  if ((int64_t)nlon * nlat * nmember >= ((int64_t)1 << 31) - kBlock) return 0;
+  return ws_layout(make_grid(nlon, nlat, nmember), nmember).total;
 }
 
 int speedy_workspace_init(void* workspace, size_t workspace_bytes, void* stream) {
@@ -366,6 +535,19 @@ int speedy_workspace_init(void* workspace, size_t workspace_bytes, void* stream)
                      (hipStream_t)stream, (uint32_t*)workspace, n);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: fill failed: ") + hipGetErrorString(e));
+  return SPEEDY_OK;
+}
+
+int speedy_workspace_status(void* workspace, size_t workspace_bytes, uint32_t* status, void* stream) {
+  if (!workspace || workspace_bytes < 256 || !status) return fail(SPEEDY_ERR_ARG, "speedy: bad arguments");
+  uint32_t* err = static_cast<uint32_t*>(workspace) + 1;   // word 1: KArgs::err
+  hipError_t e = hipMemcpyAsync(status, err, sizeof(uint32_t), hipMemcpyDeviceToHost, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  if (e == hipSuccess && *status) {
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, err, (int64_t)1);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: status read failed: ") + hipGetErrorString(e));
   return SPEEDY_OK;
 }
 
@@ -395,18 +577,18 @@ int speedy_fwd(const SpeedyFields* fields, const SpeedyVertical* vert, const Spe
   g_last_error.clear();
   hipStream_t s = (hipStream_t)stream;
   switch (vert->kx) {
-    case 7: return launch_fwd<7>(a, s);
-    case 8: return launch_fwd<8>(a, s);
-    default: return launch_fwd<16>(a, s);
+    case 7: return launch_fwd_kx<7>(a, s);
+    case 8: return launch_fwd_kx<8>(a, s);
+    default: return launch_fwd_kx<16>(a, s);
   }
 }
 
 int speedy_read_level_sets(const void* workspace, size_t workspace_bytes, int32_t nmember, uint32_t* out,
                            void* stream) {
   if (!workspace || !out || nmember <= 0) return fail(SPEEDY_ERR_ARG, "speedy: bad arguments");
-  const WsLayout L = ws_layout(1, nmember);
-  if (workspace_bytes < L.lat) return fail(SPEEDY_ERR_WORKSPACE, "speedy: workspace too small");
-  hipError_t e = hipMemcpyAsync(out, static_cast<const char*>(workspace) + L.s_final, sizeof(uint32_t) * nmember,
+  if (workspace_bytes < 256 + align256(sizeof(uint32_t) * (size_t)nmember))
+    return fail(SPEEDY_ERR_WORKSPACE, "speedy: workspace too small");
+  hipError_t e = hipMemcpyAsync(out, static_cast<const char*>(workspace) + 256, sizeof(uint32_t) * nmember,
                                 hipMemcpyDeviceToHost, (hipStream_t)stream);
   if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: copy failed: ") + hipGetErrorString(e));

@@ -85,9 +85,11 @@ def load(path: str | None = None):
     lib = ctypes.CDLL(p)
     lib.speedy_abi_version.restype = ctypes.c_int
     lib.speedy_workspace_bytes.restype = ctypes.c_size_t
-    lib.speedy_workspace_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    lib.speedy_workspace_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.speedy_workspace_init.restype = ctypes.c_int
     lib.speedy_workspace_init.argtypes = [CFP, ctypes.c_size_t, CFP]
+    lib.speedy_workspace_status.restype = ctypes.c_int
+    lib.speedy_workspace_status.argtypes = [CFP, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32), CFP]
     lib.speedy_diag_rows.restype = ctypes.c_int
     lib.speedy_diag_rows.argtypes = [ctypes.c_int32]
     lib.speedy_diag_names.restype = ctypes.c_int

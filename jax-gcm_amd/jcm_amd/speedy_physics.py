@@ -112,11 +112,13 @@ class SpeedyPhysics(Physics):
             self._abi_params_src = repr(self.parameters)
         return self._abi_params
 
-    def workspace(self, nlat: int, nmember: int) -> torch.Tensor:
-        key = (nlat, nmember)
+    def workspace(self, nlon: int, nlat: int, nmember: int) -> torch.Tensor:
+        key = (nlon, nlat, nmember)
         ws = self._workspaces.get(key)
         if ws is None:
-            nbytes = self._lib.speedy_workspace_bytes(nlat, nmember)
+            nbytes = self._lib.speedy_workspace_bytes(nlon, nlat, nmember)
+            if nbytes == 0:
+                raise _native.SpeedyError(f"grid {nlon}x{nlat}x{nmember} is not supported")
             ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
             _native.check(self._lib.speedy_workspace_init(ws.data_ptr(), nbytes, self._stream()),
                           "speedy_workspace_init")
@@ -200,7 +202,7 @@ class SpeedyPhysics(Physics):
             f.diag = diag.data_ptr()
         else:
             f.diag = None
-        ws = self.workspace(il, nmember)
+        ws = self.workspace(ix, il, nmember)
         flags = _native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0
         rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(geometry.to_abi()), ctypes.byref(self._params()),
                                   ctypes.byref(date.to_abi()), ix, il, nmember, flags, term_mask, level_set,
@@ -236,7 +238,7 @@ class SpeedyPhysics(Physics):
         adj.g_u, adj.g_v, adj.g_t, adj.g_q = (g.data_ptr() for g in grads)
         adj.g_ps = g_ps.data_ptr()
         adj.g_params = g_par.data_ptr()
-        ws = self.workspace(il, nmember)
+        ws = self.workspace(ix, il, nmember)
         flags = _native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0
         rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(geometry.to_abi()),
                                       ctypes.byref(self._params()), ctypes.byref(date.to_abi()), ix, il, nmember,
@@ -246,13 +248,22 @@ class SpeedyPhysics(Physics):
         pg = {f"{g}.{n}": g_par[..., i] for i, (g, n) in enumerate(VJP_PARAMS)}
         return PhysicsTendency(*out), sg, pg
 
-    def level_sets(self, nlat: int, nmember: int = 1) -> list:
-        """Q1 level set of each member in the last call with this (nlat, nmember)."""
-        ws = self.workspace(nlat, nmember)
+    def level_sets(self, nlon: int, nlat: int, nmember: int = 1) -> list:
+        """Q1 level set of each member in the last call on this grid."""
+        ws = self.workspace(nlon, nlat, nmember)
         buf = (ctypes.c_uint32 * nmember)()
         _native.check(self._lib.speedy_read_level_sets(ws.data_ptr(), ws.numel(), nmember, buf, self._stream()),
                       "speedy_read_level_sets")
         return list(buf)
+
+    def barrier_status(self, nlon: int, nlat: int, nmember: int = 1) -> int:
+        """Nonzero if a call on this grid since the last query gave up waiting for the
+        level set (its results are invalid); reading clears the flag."""
+        ws = self.workspace(nlon, nlat, nmember)
+        st = ctypes.c_uint32(0)
+        _native.check(self._lib.speedy_workspace_status(ws.data_ptr(), ws.numel(), ctypes.byref(st), self._stream()),
+                      "speedy_workspace_status")
+        return int(st.value)
 
     def get_empty_data(self, geometry: Geometry):
         """speedy_physics.py:129-133: an all-zero data struct for accumulating averages."""

@@ -16,17 +16,14 @@ static void run(const float* const* in, float* const* out, const SpeedyVertical*
   const int64_t ncol = (int64_t)nlon * nlat;
   // in: u v t q phi ps alb0 sice snowc soilw stl sst fmask orog phis0 sia coa
   const float *u = in[0], *vv = in[1], *t = in[2], *q = in[3], *phi = in[4], *ps = in[5];
+  Uniforms un;
+  make_uniforms(*p, *v, *d, un);
   float lat[6][1024];
   for (int j = 0; j < nlat && j < 1024; ++j) {
     float z[6];
-    zonal_fields(d->tyear, in[15][j], in[16][j], z);
+    zonal_column(un.zon, in[15][j], in[16][j], z);
     for (int i = 0; i < 6; ++i) lat[i][j] = z[i];
   }
-  Uniforms<KX> un;
-  make_uniforms<KX>(*p, *v, *d, un);
-  int32_t ms = d->model_step % 3;
-  if (ms < 0) ms += 3;
-  flags |= (ms == 0) ? (1u << 16) : 0u;
   auto load = [&](int64_t c, ColumnIn<KX>& col) {
     for (int k = 0; k < KX; ++k) {
       col.T[k] = t[k * ncol + c]; col.Q[k] = q[k * ncol + c]; col.PHI[k] = phi[k * ncol + c];
@@ -54,7 +51,8 @@ static void run(const float* const* in, float* const* out, const SpeedyVertical*
     PhaseA<KX> a;
     phase_a<KX>(col, *p, *v, un, a);
     PhaseB<KX> b;
-    phase_b<KX, true>(col, a, S, *p, *v, un, flags, b);
+    phase_b1<KX>(col, a, *p, *v, un, b);
+    phase_b2<KX, true>(col, a, S, *p, *v, un, flags, b);
     for (int k = 0; k < KX; ++k) {
       float tt = 0.f, qq = 0.f, du = 0.f, dv = 0.f;
       if (term_mask & (1u << SPEEDY_TERM_CONVECTION)) { tt = tt + a.tconv[k]; qq = qq + a.qconv[k]; }

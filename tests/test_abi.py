@@ -37,8 +37,9 @@ def test_struct_layouts():
 def test_workspace_and_diag_queries():
     lib = _native.load()
     assert lib.speedy_abi_version() == 1
-    assert lib.speedy_workspace_bytes(128, 1) > 0
-    assert lib.speedy_workspace_bytes(0, 1) == 0
+    assert lib.speedy_workspace_bytes(256, 128, 1) > 0
+    assert lib.speedy_workspace_bytes(0, 128, 1) == 0
+    assert lib.speedy_workspace_bytes(1280, 640, 4096) == 0   # more than 2^31 columns
     for kx in (7, 8, 16):
         names = _native.diag_names(kx)
         assert len(names) == lib.speedy_diag_rows(kx)

@@ -50,13 +50,14 @@ def check_fwd(phys, case, params=None, tol=H.REL_TOL, label="", strict=True):
         print(f"{label} {f}: rel err vs fp32 oracle {e32:.2e}, vs fp64 oracle {e64:.2e} "
               f"(fp32 oracle vs fp64 {eref:.2e})", flush=True)
     kx, ix, il = case[2].nodal_shape
-    S = phys.level_sets(il, 1)[0]
+    S = phys.level_sets(ix, il, 1)[0]
     assert S == S_ref, f"level set {bin(S)} != {bin(S_ref)}"
     np.testing.assert_array_equal(_np(data.convection.iptop).astype(np.int32), rdata.convection.iptop)
     np.testing.assert_array_equal(_np(data.shortwave_rad.icltop).astype(np.int32), rdata.shortwave_rad.icltop)
     for f, (ok, e32) in errs.items():
         good = (e32 <= tol) if strict else ok
         assert good, f"{label} {f}: {e32:.3e}"
+    assert phys.barrier_status(ix, il, 1) == 0, "level-set wait gave up"
     check_fwd.rdata64 = rdata64
     return tend, data, rdata
 
@@ -178,7 +179,7 @@ def test_fwd_members_stacked(phys):
     st = st.copy(specific_humidity=q)
     tend, _ = phys.compute_tendencies(st, fo, ge, da)
     torch.cuda.synchronize()
-    S = phys.level_sets(32, 3)
+    S = phys.level_sets(64, 32, 3)
     for m in range(3):
         ref, _, S_ref = H.run_oracle(st, fo, ge, da, member=m)
         ref64, _, _ = H.run_oracle(st, fo, ge, da, member=m, dtype=np.float64)
@@ -200,7 +201,7 @@ def test_fwd_repeated_calls_reset_level_set(phys):
         phys.compute_tendencies(*case)
         torch.cuda.synchronize()
         _, _, S_ref = H.run_oracle(*case)
-        S = phys.level_sets(32, 1)[0]
+        S = phys.level_sets(64, 32, 1)[0]
         assert S == S_ref, (bin(S), bin(S_ref))
 
 
@@ -234,3 +235,42 @@ def test_fwd_convection_fortran_fixture(phys):
     qt = dfqa.copy(); qt[1:] = dfqa[1:] * ge.grdsig[1:, None, None]
     np.testing.assert_allclose(_np(tend.temperature), tt, atol=1e-4, rtol=0)
     np.testing.assert_allclose(_np(tend.specific_humidity), qt, atol=1e-4, rtol=0)
+
+
+@pytest.mark.parametrize("nmember", [1, 3])
+def test_fwd_graph_replay_matches_eager(nmember):
+    """Calls captured in a HIP graph (plain launch of the fused kernel) give, on every
+    replay, exactly the eager call's tendencies; eager work runs between replays."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    st, fo, ge, da = synthetic.make_case(128, 64, 8, seed=4, land=True, nmember=nmember, member_t_sigma=1.0)
+    st = st.copy(specific_humidity=(st.specific_humidity * np.float32(1.1)).astype(np.float32))
+    dev = phys.device
+    st = st.copy(**{k: torch.as_tensor(np.ascontiguousarray(getattr(st, k)), device=dev)
+                    for k in ("u_wind", "v_wind", "temperature", "specific_humidity", "geopotential",
+                              "normalized_surface_pressure")})
+    eager, _ = phys.compute_tendencies(st, fo, ge, da)
+    eager = [getattr(eager, f).clone() for f in FIELDS]
+    S_eager = phys.level_sets(128, 64, nmember)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        phys.compute_tendencies(st, fo, ge, da)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        outs = [phys.compute_tendencies(st, fo, ge, da)[0] for _ in range(3)]
+    for rep in range(3):
+        for o in outs:
+            for f in FIELDS:
+                getattr(o, f).fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        phys.compute_tendencies(st, fo, ge, da)   # eager work between replays
+        torch.cuda.synchronize()
+        for o in outs:
+            for f, e in zip(FIELDS, eager):
+                assert torch.equal(getattr(o, f), e), (rep, f)
+        assert phys.level_sets(128, 64, nmember) == S_eager
+        assert phys.barrier_status(128, 64, nmember) == 0
