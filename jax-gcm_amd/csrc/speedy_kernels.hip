@@ -48,7 +48,7 @@ struct KArgs {
   SpeedyFields f;
   SpeedyVertical v;
   SpeedyParams p;
-  Uniforms u;
+  Uniforms<float> u;
   uint32_t ncol, ntot, nmember, nlat;
   uint32_t nblk;                     // blocks of kBlock columns
   uint32_t ls_blocks, chunk, jmax;   // two-kernel level-set pass: chunks of `chunk` columns
@@ -94,7 +94,7 @@ __device__ __forceinline__ uint32_t divq(uint32_t n, const Divisor& dv) {
 }
 
 template <int KX>
-__device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t c, ColumnIn<KX>& col) {
+__device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t c, ColumnIn<float, KX>& col) {
   const size_t ncol = a.ncol;
   const size_t base3 = (size_t)m * KX * ncol + c;
 #pragma unroll
@@ -118,7 +118,7 @@ __device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t 
 }
 
 template <int KX>
-__device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<KX>& col) {
+__device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<float, KX>& col) {
   const uint32_t j = c - divq(c, a.dlat) * a.nlat;
   float z[6];
   zonal_column(a.u.zon, a.f.sia[j], a.f.coa[j], z);
@@ -126,8 +126,8 @@ __device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<
 }
 
 template <int KX, bool DIAG>
-__device__ __forceinline__ void store_outputs(const KArgs& a, uint32_t m, uint32_t c, const ColumnIn<KX>& col,
-                                              const PhaseA<KX>& pa, const PhaseB<KX>& pb) {
+__device__ __forceinline__ void store_outputs(const KArgs& a, uint32_t m, uint32_t c, const ColumnIn<float, KX>& col,
+                                              const PhaseA<float, KX>& pa, const PhaseB<float, KX>& pb) {
   // term-ordered sum (speedy_physics.py:123-125)
   const uint32_t tm = a.term_mask;
   const size_t ncol = a.ncol;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
   for (uint32_t gid = c_begin + threadIdx.x; gid < c_end; gid += kBlock) {
     const uint32_t m = divq(gid, a.dcol);
     const uint32_t c = gid - m * a.ncol;
-    ColumnIn<KX> col;
+    ColumnIn<float, KX> col;
     const size_t ncol = a.ncol;
     const size_t base3 = (size_t)m * KX * ncol + c;
 #pragma unroll
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
       col.PHI[k] = a.f.phi[base3 + k * ncol];
     }
     col.ps = a.f.ps[(size_t)m * ncol + c];
-    atomicOr(&s_or[m - m_first], clip_bit(cloud_top<KX>(col, a.p, a.v, a.u), KX));
+    atomicOr(&s_or[m - m_first], clip_bit(cloud_top<float, KX>(col, a.p, a.v, a.u), KX));
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[blockIdx.x * a.jmax + i] = s_or[i];
@@ -276,10 +276,10 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
   const uint32_t blk0 = blockIdx.x * kBlock;
   const uint32_t m_first = divq(blk0, a.dcol);
   const uint32_t nm = divq(min(blk0 + kBlock, a.ntot) - 1u, a.dcol) - m_first + 1u;   // <= 2 when fused
-  ColumnIn<KX> col;
+  ColumnIn<float, KX> col;
   load_state<KX>(a, m, c, col);
-  PhaseA<KX> pa;
-  phase_a<KX>(col, a.p, a.v, a.u, pa);
+  PhaseA<float, KX> pa;
+  phase_a<float, KX>(col, a.p, a.v, a.u, pa);
   const bool need_s = a.level_set < 0;
   if (FUSED && need_s) {
     // publish this block's word(s) for the call's level set
@@ -295,8 +295,8 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
         __hip_atomic_store(&a.partial[2u * blockIdx.x + j], (s_tag << 16) | s_or[j], __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
   }
-  PhaseB<KX> pb;
-  phase_b1<KX>(col, pa, a.p, a.v, a.u, pb);
+  PhaseB<float, KX> pb;
+  phase_b1<float, KX>(col, pa, a.p, a.v, a.u, pb);
   uint32_t S = (uint32_t)a.level_set;
   const bool wide = a.ncol >= (uint32_t)kBlock;   // a block spans at most two members
   if (need_s) {
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
     }
   }
   load_zonal<KX>(a, c, col);
-  phase_b2<KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
+  phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
   if (active) store_outputs<KX, DIAG>(a, m, c, col, pa, pb);
 }
 

@@ -9,22 +9,22 @@
 
 using namespace speedy;
 
-template <int KX>
-static void run(const float* const* in, float* const* out, const SpeedyVertical* v, const SpeedyParams* p,
+template <typename R, int KX>
+static void run(const R* const* in, R* const* out, const SpeedyVertical* v, const SpeedyParams* p,
                 const SpeedyDate* d, int nlon, int nlat, uint32_t flags, uint32_t term_mask, int32_t level_set,
                 uint32_t* s_out) {
   const int64_t ncol = (int64_t)nlon * nlat;
   // in: u v t q phi ps alb0 sice snowc soilw stl sst fmask orog phis0 sia coa
-  const float *u = in[0], *vv = in[1], *t = in[2], *q = in[3], *phi = in[4], *ps = in[5];
-  Uniforms un;
+  const R *u = in[0], *vv = in[1], *t = in[2], *q = in[3], *phi = in[4], *ps = in[5];
+  Uniforms<R> un;
   make_uniforms(*p, *v, *d, un);
-  float lat[6][1024];
+  R lat[6][1024];
   for (int j = 0; j < nlat && j < 1024; ++j) {
-    float z[6];
+    R z[6];
     zonal_column(un.zon, in[15][j], in[16][j], z);
     for (int i = 0; i < 6; ++i) lat[i][j] = z[i];
   }
-  auto load = [&](int64_t c, ColumnIn<KX>& col) {
+  auto load = [&](int64_t c, ColumnIn<R, KX>& col) {
     for (int k = 0; k < KX; ++k) {
       col.T[k] = t[k * ncol + c]; col.Q[k] = q[k * ncol + c]; col.PHI[k] = phi[k * ncol + c];
     }
@@ -37,24 +37,24 @@ static void run(const float* const* in, float* const* out, const SpeedyVertical*
   };
   uint32_t S = 0;
   for (int64_t c = 0; c < ncol; ++c) {
-    ColumnIn<KX> col;
+    ColumnIn<R, KX> col;
     load(c, col);
-    int icl = cloud_top<KX>(col, *p, *v, un);
+    int icl = cloud_top<R, KX>(col, *p, *v, un);
     icl = icl < 0 ? 0 : (icl > KX - 1 ? KX - 1 : icl);
     S |= 1u << icl;
   }
   *s_out = S;
   if (level_set >= 0) S = (uint32_t)level_set;
   for (int64_t c = 0; c < ncol; ++c) {
-    ColumnIn<KX> col;
+    ColumnIn<R, KX> col;
     load(c, col);
-    PhaseA<KX> a;
-    phase_a<KX>(col, *p, *v, un, a);
-    PhaseB<KX> b;
-    phase_b1<KX>(col, a, *p, *v, un, b);
-    phase_b2<KX, true>(col, a, S, *p, *v, un, flags, b);
+    PhaseA<R, KX> a;
+    phase_a<R, KX>(col, *p, *v, un, a);
+    PhaseB<R, KX> b;
+    phase_b1<R, KX>(col, a, *p, *v, un, b);
+    phase_b2<R, KX, true>(col, a, S, *p, *v, un, flags, b);
     for (int k = 0; k < KX; ++k) {
-      float tt = 0.f, qq = 0.f, du = 0.f, dv = 0.f;
+      R tt = 0, qq = 0, du = 0, dv = 0;
       if (term_mask & (1u << SPEEDY_TERM_CONVECTION)) { tt = tt + a.tconv[k]; qq = qq + a.qconv[k]; }
       if (term_mask & (1u << SPEEDY_TERM_CONDENSATION)) { tt = tt + a.tlsc[k]; qq = qq + a.qlsc[k]; }
       if (term_mask & (1u << SPEEDY_TERM_SHORTWAVE)) tt = tt + b.tsw[k];
@@ -66,21 +66,36 @@ static void run(const float* const* in, float* const* out, const SpeedyVertical*
       out[0][k * ncol + c] = du; out[1][k * ncol + c] = dv; out[2][k * ncol + c] = tt; out[3][k * ncol + c] = qq;
     }
     // a few diagnostics: iptop, icltop, cbmf, precnv, precls, cloudc, clstr, rsds, rlds, tskin
-    float* dg = out[4];
-    const float vals[] = {(float)a.iptop, (float)a.icltop, a.cbmf, a.precnv, a.precls, a.cloudc, a.clstr,
+    R* dg = out[4];
+    const R vals[] = {(R)a.iptop, (R)a.icltop, a.cbmf, a.precnv, a.precls, a.cloudc, a.clstr,
                           b.rsds, b.rlds, b.tskin};
     for (int i = 0; i < 10; ++i) dg[i * ncol + c] = vals[i];
   }
 }
 
+template <typename R>
+static int dispatch(int kx, const R* const* in, R* const* out, const SpeedyVertical* v, const SpeedyParams* p,
+                    const SpeedyDate* d, int nlon, int nlat, uint32_t flags, uint32_t term_mask, int32_t level_set,
+                    uint32_t* s_out) {
+  if (nlat > 1024) return 1;
+  switch (kx) {
+    case 7: run<R, 7>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
+    case 8: run<R, 8>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
+    case 16: run<R, 16>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
+    default: return 2;
+  }
+}
+
+// in: u v t q phi ps alb0 sice snowc soilw stl sst fmask orog phis0 sia coa;
+// out: du dv dt dq, then 10 diagnostic rows.  float32 and float64 builds.
 extern "C" int column_host_fwd(int kx, const float* const* in, float* const* out, const SpeedyVertical* v,
                                const SpeedyParams* p, const SpeedyDate* d, int nlon, int nlat, uint32_t flags,
                                uint32_t term_mask, int32_t level_set, uint32_t* s_out) {
-  if (nlat > 1024) return 1;
-  switch (kx) {
-    case 7: run<7>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
-    case 8: run<8>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
-    case 16: run<16>(in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out); return 0;
-    default: return 2;
-  }
+  return dispatch<float>(kx, in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out);
+}
+
+extern "C" int column_host_fwd64(int kx, const double* const* in, double* const* out, const SpeedyVertical* v,
+                                 const SpeedyParams* p, const SpeedyDate* d, int nlon, int nlat, uint32_t flags,
+                                 uint32_t term_mask, int32_t level_set, uint32_t* s_out) {
+  return dispatch<double>(kx, in, out, v, p, d, nlon, nlat, flags, term_mask, level_set, s_out);
 }

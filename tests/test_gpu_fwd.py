@@ -274,3 +274,32 @@ def test_fwd_graph_replay_matches_eager(nmember):
                 assert torch.equal(getattr(o, f), e), (rep, f)
         assert phys.level_sets(128, 64, nmember) == S_eager
         assert phys.barrier_status(128, 64, nmember) == 0
+
+
+@pytest.mark.parametrize("nlon,nlat,nmember", [(256, 128, 8), (8, 4, 6)])
+def test_fwd_two_kernel_level_set_path(nlon, nlat, nmember):
+    """Grids beyond one co-resident launch (T85 x 8 members = 262144 columns) and
+    members narrower than a workgroup (8 x 4 columns) take the two-launch path
+    (level-set kernel, then forward); every member matches its own oracle run."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=9, land=True, nmember=nmember, member_t_sigma=1.0)
+    q = st.specific_humidity.copy()
+    q[1::2] *= np.float32(1.12)   # odd members condense widely: member-specific level sets
+    st = st.copy(specific_humidity=q)
+    phys._lib.speedy_timer_enable(8)
+    tend, _ = phys.compute_tendencies(st, fo, ge, da)
+    torch.cuda.synchronize()
+    from jcm_amd import _native
+    launches = _native.timer_read()
+    phys._lib.speedy_timer_disable()
+    assert launches["levelset"][1] == 1 and launches["fwd"][1] == 1, launches
+    S = phys.level_sets(nlon, nlat, nmember)
+    for m in range(nmember):
+        ref, _, S_ref = H.run_oracle(st, fo, ge, da, member=m)
+        ref64, _, _ = H.run_oracle(st, fo, ge, da, member=m, dtype=np.float64)
+        assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
+        for f in FIELDS:
+            ok, e32, e64, eref = H.parity(_np(getattr(tend, f)[m]), getattr(ref, f), getattr(ref64, f))
+            assert ok, f"member {m} {f}: {e32:.3e} (fp64 {e64:.2e}, oracle {eref:.2e})"

@@ -99,3 +99,43 @@ def test_host_column_level_set_override():
     assert H.rel_inf(outs[2], tend.temperature) <= H.REL_TOL
     ref_free, _, _ = H.run_oracle(*case, params=p)
     assert H.rel_inf(tend.temperature, ref_free.temperature) > 1e-4   # S matters
+
+
+F64_TOL = 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_host_column_float64_matches_float64_oracle(name):
+    """The same column routine instantiated in double against the float64 oracle
+    fed the library's float32 parameters and per-level constants: every expression
+    of the device code is pinned to the oracle up to operation-order rounding
+    (bound 1e-12 per field, measured ~1e-14), including the discrete decisions."""
+    case = make(name)
+    p = synthetic.draw_vjp_parameters(2) if CASES[name]["wet"] else Parameters.default()
+    outs, diag, S = H.run_host_column(case, p, dtype=np.float64)
+    tend, data, S_ref = H.run_oracle64_same_constants(case, p)
+    assert S == S_ref
+    np.testing.assert_array_equal(diag["iptop"], data.convection.iptop)
+    np.testing.assert_array_equal(diag["icltop"], data.shortwave_rad.icltop)
+    for f, a in zip(FIELDS, outs):
+        e = H.rel_inf(a, getattr(tend, f))
+        assert e <= F64_TOL, f"{f}: {e:.2e}"
+    for k, ref in (("cbmf", data.convection.cbmf), ("precnv", data.convection.precnv),
+                   ("precls", data.condensation.precls), ("cloudc", data.shortwave_rad.cloudc),
+                   ("clstr", data.shortwave_rad.cloudstr), ("rsds", data.shortwave_rad.rsds),
+                   ("rlds", data.surface_flux.rlds), ("tskin", data.surface_flux.tskin)):
+        assert H.rel_inf(diag[k], ref) <= F64_TOL, k
+
+
+@pytest.mark.parametrize("term", [3, 4, 6, 8, 9, 10])
+def test_host_column_float64_single_terms(term):
+    case = make("T21_wet")
+    p = Parameters.default()
+    outs, _, _ = H.run_host_column(case, p, term_mask=1 << term, dtype=np.float64)
+    tend, _, _ = H.run_oracle64_same_constants(case, p, term_mask=1 << term)
+    for f, a in zip(FIELDS, outs):
+        ref = getattr(tend, f)
+        if np.max(np.abs(ref)) == 0:
+            assert np.max(np.abs(a)) == 0, f
+        else:
+            assert H.rel_inf(a, ref) <= F64_TOL, f
