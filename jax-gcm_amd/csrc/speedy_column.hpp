@@ -45,6 +45,13 @@ SPD_HD double mcos(double x) { return cos(x); }
 SPD_HD float masin(float x) { return asinf(x); }
 SPD_HD double masin(double x) { return asin(x); }
 
+// Derivatives of max(x, y) and min(x, y) with respect to x, with JAX's rule
+// for ties (lax.max/lax.min JVP: half to each operand when x == y).
+template <typename R>
+SPD_HD R dmax_x(R x, R y) { return x > y ? R(1.0) : (x == y ? R(0.5) : R(0.0)); }
+template <typename R>
+SPD_HD R dmin_x(R x, R y) { return x < y ? R(1.0) : (x == y ? R(0.5) : R(0.0)); }
+
 // jcm/constants.py:3-5, jcm/physics/speedy/physical_constants.py:8-45.  Python
 // double expressions are evaluated in double and rounded once, as JAX does
 // when a Python R meets a float32 array.
@@ -149,12 +156,13 @@ struct Uniforms {
   R entr[SPEEDY_KX_MAX], fm0, rdps;
   // large-scale condensation (large_scale_condensation.py:48-63,77)
   R rtlsc, rhref[SPEEDY_KX_MAX], dqmax[SPEEDY_KX_MAX], pfact[SPEEDY_KX_MAX];
+  R rhref_d[SPEEDY_KX_MAX];       // d rhref / d rhlsc (0, 1/2 at the tie, 1)
   // clouds (shortwave_radiation.py:348,387)
   R rrcl, rgse;
   // shortwave absorptivity of dry air + aerosol per level (shortwave_radiation.py:82,93)
   R sw_abs1[SPEEDY_KX_MAX];
   // longwave (speedy/forcing.py:33; shortwave_radiation.py:203; longwave_radiation.py:100,156)
-  R ablco2, eps1, omeps, refsfc, corlw;
+  R ablco2, ablco2_d, eps1, omeps, refsfc, corlw;   // ablco2_d = d ablco2 / d co2_scale
   // surface fluxes (surface_flux.py:75-76,101,135-137,149,154,253,311)
   R esbc, ghum0, gtemp0, astab, rdth, rhdrag, cdl, cds, chlcp, chscp;
   // vertical diffusion (vertical_diffusion.py:45-63,99-118)
@@ -190,7 +198,9 @@ inline void make_uniforms(const SpeedyParams& p, const SpeedyVertical& v, const 
   u.rtlsc = R(1.0) / (p.trlsc * R(3600.0));
   for (int k = 0; k < kx; ++k) {
     const R sig2 = sq(R(v.fsg[k]));
-    u.rhref[k] = mmax(p.rhlsc + p.drhlsc * (sig2 - R(1.0)), p.rhblsc);
+    const R rharg = p.rhlsc + p.drhlsc * (sig2 - R(1.0));
+    u.rhref[k] = mmax(rharg, p.rhblsc);
+    u.rhref_d[k] = dmax_x(rharg, R(p.rhblsc));
     u.dqmax[k] = R(10.0) * sig2 * u.rtlsc;
     u.pfact[k] = v.dhs[k] * R(PRG);
     u.sw_abs1[k] = p.absdry + p.absaer * sq(R(v.fsg[k]));
@@ -198,6 +208,7 @@ inline void make_uniforms(const SpeedyParams& p, const SpeedyVertical& v, const 
   u.rrcl = R(1.0) / (R(p.rhcl2) - p.rhcl1);
   u.rgse = R(1.0) / (R(p.gse_s1) - p.gse_s0);
   R ab = R(6.0) * mexp((R)p.increase_co2 * R(0.005) * ((R)d.model_year + d.tyear - (R)p.co2_year_ref));
+  u.ablco2_d = ab;
   if (p.co2_scale != R(1.0)) ab = ab * p.co2_scale;
   u.ablco2 = ab;
   u.eps1 = p.epslw / (R(v.dhs[0]) + v.dhs[1]);
@@ -258,195 +269,253 @@ inline void make_uniforms(const SpeedyParams& p, const SpeedyVertical& v, const 
 }
 
 // ---------------------------------------------------------------------------
-// Phase A.
+// Terms.  Each term is a function that computes it from its inputs and records
+// in a tape struct the intermediates its adjoint (speedy_adjoint.hpp) needs.  In
+// the forward kernels the tapes are dead stores that the compiler removes; the
+// adjoint recomputes a term right before reversing it instead of keeping the
+// whole forward alive.
 // ---------------------------------------------------------------------------
+
+// humidity (humidity.py:40-46) and the dry static energy of convection.py:121.
 template <typename R, int KX>
-SPD_HD void phase_a(const ColumnIn<R, KX>& c, const SpeedyParams& p, const SpeedyVertical& v,
-                    const Uniforms<R>& u, PhaseA<R, KX>& a) {
-  // humidity (humidity.py:40-46)
+SPD_HD void humidity(const ColumnIn<R, KX>& c, const SpeedyVertical& v, PhaseA<R, KX>& a) {
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
     a.qsat[k] = qsat_f(c.T[k], c.ps, R(v.fsg[k]));
     a.rh[k] = c.Q[k] / a.qsat[k];
   }
-  // ---- convection (convection.py:99-237)
 #pragma unroll
   for (int k = 0; k < KX; ++k) a.se[k] = R(CP) * c.T[k] + c.PHI[k];
+}
+
+template <typename R, int KX>
+struct ConvTape {
+  int ktop2;
+  bool case1, case2;
+  R qthr0, mse0, msthr, qdif, qmax, qb, sb, fpsa, fmass;
+  R sbh[KX], qbh[KX], en[KX], FM[KX], FUS[KX], FUQ[KX], FDS[KX], FDQ[KX], delq[KX];
+  R fm_t, fus_t, fuq_t, fds_t, fdq_t, qsatb;
+  R dfse[KX], dfqa[KX];
+};
+
+// convection (convection.py:15-237): diagnose_convection and the fluxes.
+template <typename R, int KX>
+SPD_HD void convection(const ColumnIn<R, KX>& c, const SpeedyParams& p, const SpeedyVertical& v,
+                       const Uniforms<R>& u, PhaseA<R, KX>& a, ConvTape<R, KX>& t) {
   // diagnose_convection (:15-97)
-  R mss2[KX];
+  R mss[KX], mss2[KX];
+#pragma unroll
+  for (int k = 0; k < KX; ++k) mss[k] = a.se[k] + R(ALHC) * a.qsat[k];
+#pragma unroll
+  for (int k = 0; k < KX - 1; ++k) mss2[k] = mss[k] + u.wvi1[k] * (mss[k + 1] - mss[k]);
+  mss2[KX - 1] = R(0.0);
+  const R mse0 = a.se[KX - 1] + R(ALHC) * c.Q[KX - 1];
+  const R mse1 = mmin(mse0, a.se[KX - 2] + R(ALHC) * c.Q[KX - 2]);
+  const R mss0 = mmax(mse0, mss[KX - 1]);
+  int ktop1 = KX, ktop2 = KX;
+#pragma unroll
+  for (int k = KX - 4; k >= 2; --k) {   // first unstable level from the top wins
+    if (mss0 > mss2[k]) ktop1 = k + 1;
+    if (mse1 > mss2[k]) ktop2 = k + 1;
+  }
+  R msthr = mss2[KX - 1];
+#pragma unroll
+  for (int k = 2; k <= KX - 4; ++k)
+    if (ktop2 == k + 1) msthr = mss2[k];
+  const R qthr0 = p.rhbl * a.qsat[KX - 1];
+  const R qthr1 = p.rhbl * a.qsat[KX - 2];
+  const bool lqthr = (c.Q[KX - 1] > qthr0) && (c.Q[KX - 2] > qthr1);
+  const bool mask_psa = c.ps > p.psmin;
+  const bool case1 = mask_psa && (ktop1 < KX) && (ktop2 < KX);
+  const bool case2 = mask_psa && (ktop1 < KX) && !(ktop2 < KX) && lqthr;
+  a.iptop_conv = (case1 || case2) ? ktop1 : KX + 1;
+  R qdif = R(0.0);
+  if (case1) qdif = mmax(c.Q[KX - 1] - qthr0, (mse0 - msthr) * R(RLHC));
+  if (case2) qdif = c.Q[KX - 1] - qthr0;
+  t.ktop2 = ktop2; t.case1 = case1; t.case2 = case2;
+  t.qthr0 = qthr0; t.mse0 = mse0; t.msthr = msthr; t.qdif = qdif;
+  // mass fluxes
+  const int iptop = a.iptop_conv;
+  const bool mask = iptop < KX;
+  const R qmax = mmax(R(1.01) * c.Q[KX - 1], a.qsat[KX - 1]);
+  R* sbh = t.sbh;
+  R* qbh = t.qbh;
+  sbh[0] = R(0.0); qbh[0] = R(0.0);
+#pragma unroll
+  for (int k = 1; k < KX; ++k) {
+    sbh[k] = a.se[k - 1] + u.wvi1[k - 1] * (a.se[k] - a.se[k - 1]);
+    qbh[k] = c.Q[k - 1] + u.wvi1[k - 1] * (c.Q[k] - c.Q[k - 1]);
+  }
+  const R sb = sbh[KX - 1];
+  const R qb = mmin(qbh[KX - 1], c.Q[KX - 1]);
+  const R fpsa = c.ps * mmin(R(1.0), (c.ps - p.psmin) * u.rdps);
+  const R fmass = u.fm0 * fpsa * mmin(R(5.0), qdif / (qmax - qb));
+  t.qmax = qmax; t.qb = qb; t.sb = sb; t.fpsa = fpsa; t.fmass = fmass;
+  a.cbmf = (mask ? R(1.0) : R(0.0)) * fmass;
+  const R fus = fmass * a.se[KX - 1], fuq = fmass * qmax;
+  const R fds = fmass * sb, fdq = fmass * qb;
+  R* dfse = t.dfse;
+  R* dfqa = t.dfqa;
+  // entrainment and cumulative upward fluxes from the bottom (:178-191)
+  R* en = t.en;
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    const bool lm = (k <= KX - 2) && (k >= iptop);
+    en[k] = (lm && k >= 1) ? u.entr[k] * c.ps * a.cbmf : R(0.0);
+  }
+  R* FM = t.FM;
+  R* FUS = t.FUS;
+  R* FUQ = t.FUQ;
   {
-    R mss[KX];
+    R r1 = R(0.0), r2 = R(0.0), r3 = R(0.0);
 #pragma unroll
-    for (int k = 0; k < KX; ++k) mss[k] = a.se[k] + R(ALHC) * a.qsat[k];
-#pragma unroll
-    for (int k = 0; k < KX - 1; ++k) mss2[k] = mss[k] + u.wvi1[k] * (mss[k + 1] - mss[k]);
-    mss2[KX - 1] = R(0.0);
-    const R mse0 = a.se[KX - 1] + R(ALHC) * c.Q[KX - 1];
-    const R mse1 = mmin(mse0, a.se[KX - 2] + R(ALHC) * c.Q[KX - 2]);
-    const R mss0 = mmax(mse0, mss[KX - 1]);
-    int ktop1 = KX, ktop2 = KX;
-#pragma unroll
-    for (int k = KX - 4; k >= 2; --k) {   // first unstable level from the top wins
-      if (mss0 > mss2[k]) ktop1 = k + 1;
-      if (mse1 > mss2[k]) ktop2 = k + 1;
-    }
-    R msthr = mss2[KX - 1];
-#pragma unroll
-    for (int k = 2; k <= KX - 4; ++k)
-      if (ktop2 == k + 1) msthr = mss2[k];
-    const R qthr0 = p.rhbl * a.qsat[KX - 1];
-    const R qthr1 = p.rhbl * a.qsat[KX - 2];
-    const bool lqthr = (c.Q[KX - 1] > qthr0) && (c.Q[KX - 2] > qthr1);
-    const bool mask_psa = c.ps > p.psmin;
-    const bool case1 = mask_psa && (ktop1 < KX) && (ktop2 < KX);
-    const bool case2 = mask_psa && (ktop1 < KX) && !(ktop2 < KX) && lqthr;
-    a.iptop_conv = (case1 || case2) ? ktop1 : KX + 1;
-    R qdif = R(0.0);
-    if (case1) qdif = mmax(c.Q[KX - 1] - qthr0, (mse0 - msthr) * R(RLHC));
-    if (case2) qdif = c.Q[KX - 1] - qthr0;
-    // mass fluxes
-    const int iptop = a.iptop_conv;
-    const bool mask = iptop < KX;
-    const R qmax = mmax(R(1.01) * c.Q[KX - 1], a.qsat[KX - 1]);
-    R sbh[KX], qbh[KX];
-    sbh[0] = R(0.0); qbh[0] = R(0.0);
-#pragma unroll
-    for (int k = 1; k < KX; ++k) {
-      sbh[k] = a.se[k - 1] + u.wvi1[k - 1] * (a.se[k] - a.se[k - 1]);
-      qbh[k] = c.Q[k - 1] + u.wvi1[k - 1] * (c.Q[k] - c.Q[k - 1]);
-    }
-    const R sb = sbh[KX - 1];
-    const R qb = mmin(qbh[KX - 1], c.Q[KX - 1]);
-    const R fpsa = c.ps * mmin(R(1.0), (c.ps - p.psmin) * u.rdps);
-    const R fmass = u.fm0 * fpsa * mmin(R(5.0), qdif / (qmax - qb));
-    a.cbmf = (mask ? R(1.0) : R(0.0)) * fmass;
-    const R fus = fmass * a.se[KX - 1], fuq = fmass * qmax;
-    const R fds = fmass * sb, fdq = fmass * qb;
-    R dfse[KX], dfqa[KX];
-    // entrainment and cumulative upward fluxes from the bottom (:178-191)
-    R en[KX];
-#pragma unroll
-    for (int k = 0; k < KX; ++k) {
-      const bool lm = (k <= KX - 2) && (k >= iptop);
-      en[k] = (lm && k >= 1) ? u.entr[k] * c.ps * a.cbmf : R(0.0);
-    }
-    R FM[KX], FUS[KX], FUQ[KX];
-    {
-      R r1 = R(0.0), r2 = R(0.0), r3 = R(0.0);
-#pragma unroll
-      for (int k = KX - 1; k >= 0; --k) {
-        r1 = (k == KX - 1) ? en[k] : r1 + en[k];
-        r2 = (k == KX - 1) ? en[k] * a.se[k] : r2 + en[k] * a.se[k];
-        r3 = (k == KX - 1) ? en[k] * c.Q[k] : r3 + en[k] * c.Q[k];
-        FM[k] = fmass + r1;
-        FUS[k] = fus + r2;
-        FUQ[k] = fuq + r3;
-      }
-    }
-    R FDS[KX], FDQ[KX];
-#pragma unroll
-    for (int k = 0; k < KX - 1; ++k) { FDS[k] = FM[k] * sbh[k]; FDQ[k] = FM[k] * qbh[k]; }
-    FDS[KX - 1] = fds; FDQ[KX - 1] = fdq;
-    dfse[KX - 1] = fds - fus;
-    dfqa[KX - 1] = fdq - fuq;
-    R sumfsq = R(0.0);
-#pragma unroll
-    for (int k = 0; k < KX - 1; ++k) {
-      const bool lm = (k >= iptop);
-      dfse[k] = lm ? ((FUS[k + 1] - FDS[k + 1]) - (FUS[k] - FDS[k])) : R(0.0);
-      dfqa[k] = lm ? ((FUQ[k + 1] - FDQ[k + 1]) - (FUQ[k] - FDQ[k])) : R(0.0);
-      // secondary moisture flux (:198-202)
-      const R delq = lm ? (p.rhil * a.qsat[k] - c.Q[k]) : R(0.0);
-      const R fsq = (delq > R(0.0)) ? p.smf * a.cbmf * delq : R(0.0);
-      dfqa[k] = dfqa[k] + fsq;
-      sumfsq = (k == 0) ? fsq : sumfsq + fsq;
-    }
-    dfqa[KX - 1] = dfqa[KX - 1] + (-sumfsq);
-    // top layer (:204-219): gather at iptop (0 beyond KX-1), scatter at iptop-1 (Q6)
-    R fm_t = R(0.0), fus_t = R(0.0), fuq_t = R(0.0), fds_t = R(0.0), fdq_t = R(0.0), qsatb = R(0.0);
-#pragma unroll
-    for (int k = 0; k < KX; ++k) {
-      if (iptop == k) { fm_t = FM[k]; fus_t = FUS[k]; fuq_t = FUQ[k]; fds_t = FDS[k]; fdq_t = FDQ[k]; }
-    }
-#pragma unroll
-    for (int k = 0; k < KX - 1; ++k)
-      if (iptop - 1 == k) qsatb = a.qsat[k] + u.wvi1[k] * (a.qsat[k + 1] - a.qsat[k]);
-    a.precnv = mmax(fuq_t - fm_t * qsatb, R(0.0));
-    const R top_s = fus_t - fds_t + R(ALHC) * a.precnv;
-    const R top_q = fuq_t - fdq_t - a.precnv;
-#pragma unroll
-    for (int k = 0; k < KX; ++k)
-      if (iptop - 1 == k) { dfse[k] = top_s; dfqa[k] = top_q; }
-    // flux -> tendency (:226-227)
-    const R rps = R(1.0) / c.ps;
-    a.tconv[0] = dfse[0];
-    a.qconv[0] = dfqa[0];
-#pragma unroll
-    for (int k = 1; k < KX; ++k) {
-      a.tconv[k] = dfse[k] * rps * v.grdscp[k];
-      a.qconv[k] = dfqa[k] * rps * v.grdsig[k];
+    for (int k = KX - 1; k >= 0; --k) {
+      r1 = (k == KX - 1) ? en[k] : r1 + en[k];
+      r2 = (k == KX - 1) ? en[k] * a.se[k] : r2 + en[k] * a.se[k];
+      r3 = (k == KX - 1) ? en[k] * c.Q[k] : r3 + en[k] * c.Q[k];
+      FM[k] = fmass + r1;
+      FUS[k] = fus + r2;
+      FUQ[k] = fuq + r3;
     }
   }
-  // ---- large-scale condensation (large_scale_condensation.py:14-86)
-  {
-    const R psa2 = c.ps * c.ps;
-    int first = 0;
-    R precls = R(0.0);
-    a.tlsc[0] = R(0.0);
-    a.qlsc[0] = R(0.0);
+  R* FDS = t.FDS;
+  R* FDQ = t.FDQ;
 #pragma unroll
-    for (int k = KX - 1; k >= 1; --k) {
-      const R dqa = u.rhref[k] * a.qsat[k] - c.Q[k];
-      if (!(dqa >= R(0.0))) first = k;         // argmin(dqa[1:] >= 0) + 1 (Q4 when none)
-      const bool neg = dqa < R(0.0);
-      const R dq = neg ? dqa * u.rtlsc : R(0.0);
-      a.qlsc[k] = dq;
-      a.tlsc[k] = neg ? R(TFACT) * mmin(-dq, u.dqmax[k] * psa2) : R(0.0);
-    }
+  for (int k = 0; k < KX - 1; ++k) { FDS[k] = FM[k] * sbh[k]; FDQ[k] = FM[k] * qbh[k]; }
+  FDS[KX - 1] = fds; FDQ[KX - 1] = fdq;
+  dfse[KX - 1] = fds - fus;
+  dfqa[KX - 1] = fdq - fuq;
+  R sumfsq = R(0.0);
 #pragma unroll
-    for (int k = 1; k < KX; ++k) {
-      const R t = u.pfact[k] * a.qlsc[k];
-      precls = (k == 1) ? t : precls + t;
-    }
-    a.precls = (R(0.0) - precls) * c.ps;
-    const int ipl = first == 0 ? 1 : first;
-    a.iptop = ipl < a.iptop_conv ? ipl : a.iptop_conv;
+  for (int k = 0; k < KX - 1; ++k) {
+    const bool lm = (k >= iptop);
+    dfse[k] = lm ? ((FUS[k + 1] - FDS[k + 1]) - (FUS[k] - FDS[k])) : R(0.0);
+    dfqa[k] = lm ? ((FUQ[k + 1] - FDQ[k + 1]) - (FUQ[k] - FDQ[k])) : R(0.0);
+    // secondary moisture flux (:198-202)
+    const R delq = lm ? (p.rhil * a.qsat[k] - c.Q[k]) : R(0.0);
+    t.delq[k] = delq;
+    const R fsq = (delq > R(0.0)) ? p.smf * a.cbmf * delq : R(0.0);
+    dfqa[k] = dfqa[k] + fsq;
+    sumfsq = (k == 0) ? fsq : sumfsq + fsq;
   }
-  // ---- clouds (shortwave_radiation.py:313-404)
-  {
-    a.gse = (a.se[KX - 2] - a.se[KX - 1]) / (c.PHI[KX - 2] - c.PHI[KX - 1]);
-    const R rrcl = u.rrcl;
-    const bool m0 = a.rh[KX - 2] > p.rhcl1;
-    const R cc0 = m0 ? a.rh[KX - 2] - p.rhcl1 : R(0.0);
-    const int icl0 = m0 ? KX - 2 : KX;
-    R cu[KX];
-    bool mk[KX];
-    R cmax = R(0.0);
+  dfqa[KX - 1] = dfqa[KX - 1] + (-sumfsq);
+  // top layer (:204-219): gather at iptop (0 beyond KX-1), scatter at iptop-1 (Q6)
+  R fm_t = R(0.0), fus_t = R(0.0), fuq_t = R(0.0), fds_t = R(0.0), fdq_t = R(0.0), qsatb = R(0.0);
 #pragma unroll
-    for (int k = 2; k <= KX - 3; ++k) {
-      const R drh = a.rh[k] - p.rhcl1;
-      mk[k] = (drh > cc0) && (c.Q[k] > p.qacl);
-      cu[k] = mk[k] ? drh : cc0;
-      cmax = (k == 2) ? cu[k] : mmax(cmax, cu[k]);
-    }
-    int icl = 0;
-#pragma unroll
-    for (int k = 2; k <= KX - 3; ++k) {        // Q5: R equality
-      const int cand = (cmax == cu[k]) ? (mk[k] ? k : icl0) : icl0;
-      icl = (k == 2) ? cand : (cand > icl ? cand : icl);
-    }
-    const R pr1 = mmin(p.pmaxcl, R(86.4) * (a.precnv + a.precls));
-    R cloudc = mmin(R(1.0), p.wpcl * msqrt(mmax(R(EPS), pr1)) + sq(mmin(R(1.0), cmax * rrcl)));
-    if (cloudc != cloudc) cloudc = R(1.0);
-    a.cloudc = cloudc;
-    a.icltop = a.iptop < icl ? a.iptop : icl;
-    a.qcloud = c.Q[KX - 2];
-    const R rgse = u.rgse;
-    const R fstab = mmin(mmax(rgse * (a.gse - p.gse_s0), R(0.0)), R(1.0));
-    R clstr = fstab * mmax(p.clsmax - R(1.2) * cloudc, R(0.0));
-    const R clstrl = mmax(clstr, p.clsminl) * a.rh[KX - 1];
-    a.clstr = clstr + c.fmask * (clstrl - clstr);
+  for (int k = 0; k < KX; ++k) {
+    if (iptop == k) { fm_t = FM[k]; fus_t = FUS[k]; fuq_t = FUQ[k]; fds_t = FDS[k]; fdq_t = FDQ[k]; }
   }
+#pragma unroll
+  for (int k = 0; k < KX - 1; ++k)
+    if (iptop - 1 == k) qsatb = a.qsat[k] + u.wvi1[k] * (a.qsat[k + 1] - a.qsat[k]);
+  t.fm_t = fm_t; t.fus_t = fus_t; t.fuq_t = fuq_t; t.fds_t = fds_t; t.fdq_t = fdq_t; t.qsatb = qsatb;
+  a.precnv = mmax(fuq_t - fm_t * qsatb, R(0.0));
+  const R top_s = fus_t - fds_t + R(ALHC) * a.precnv;
+  const R top_q = fuq_t - fdq_t - a.precnv;
+#pragma unroll
+  for (int k = 0; k < KX; ++k)
+    if (iptop - 1 == k) { dfse[k] = top_s; dfqa[k] = top_q; }
+  // flux -> tendency (:226-227)
+  const R rps = R(1.0) / c.ps;
+  a.tconv[0] = dfse[0];
+  a.qconv[0] = dfqa[0];
+#pragma unroll
+  for (int k = 1; k < KX; ++k) {
+    a.tconv[k] = dfse[k] * rps * v.grdscp[k];
+    a.qconv[k] = dfqa[k] * rps * v.grdsig[k];
+  }
+}
+
+// large-scale condensation (large_scale_condensation.py:14-86)
+template <typename R, int KX>
+SPD_HD void condensation(const ColumnIn<R, KX>& c, const Uniforms<R>& u, PhaseA<R, KX>& a) {
+  const R psa2 = c.ps * c.ps;
+  int first = 0;
+  R precls = R(0.0);
+  a.tlsc[0] = R(0.0);
+  a.qlsc[0] = R(0.0);
+#pragma unroll
+  for (int k = KX - 1; k >= 1; --k) {
+    const R dqa = u.rhref[k] * a.qsat[k] - c.Q[k];
+    if (!(dqa >= R(0.0))) first = k;         // argmin(dqa[1:] >= 0) + 1 (Q4 when none)
+    const bool neg = dqa < R(0.0);
+    const R dq = neg ? dqa * u.rtlsc : R(0.0);
+    a.qlsc[k] = dq;
+    a.tlsc[k] = neg ? R(TFACT) * mmin(-dq, u.dqmax[k] * psa2) : R(0.0);
+  }
+#pragma unroll
+  for (int k = 1; k < KX; ++k) {
+    const R t = u.pfact[k] * a.qlsc[k];
+    precls = (k == 1) ? t : precls + t;
+  }
+  a.precls = (R(0.0) - precls) * c.ps;
+  const int ipl = first == 0 ? 1 : first;
+  a.iptop = ipl < a.iptop_conv ? ipl : a.iptop_conv;
+}
+
+template <typename R, int KX>
+struct CloudTape {
+  R cu[KX];
+  uint32_t mk;                    // bit k: cu[k] is rh[k] - rhcl1 (else the level-(KX-2) value)
+  bool m0;
+  R cmax, pr1, ccraw, cl2, fstab_arg, fstab, clsx, clstr0;
+};
+
+// clouds (shortwave_radiation.py:313-404)
+template <typename R, int KX>
+SPD_HD void clouds(const ColumnIn<R, KX>& c, const SpeedyParams& p, const Uniforms<R>& u, PhaseA<R, KX>& a,
+                   CloudTape<R, KX>& t) {
+  a.gse = (a.se[KX - 2] - a.se[KX - 1]) / (c.PHI[KX - 2] - c.PHI[KX - 1]);
+  const R rrcl = u.rrcl;
+  const bool m0 = a.rh[KX - 2] > p.rhcl1;
+  const R cc0 = m0 ? a.rh[KX - 2] - p.rhcl1 : R(0.0);
+  const int icl0 = m0 ? KX - 2 : KX;
+  R* cu = t.cu;
+  bool mk[KX];
+  R cmax = R(0.0);
+  t.mk = 0u;
+#pragma unroll
+  for (int k = 2; k <= KX - 3; ++k) {
+    const R drh = a.rh[k] - p.rhcl1;
+    mk[k] = (drh > cc0) && (c.Q[k] > p.qacl);
+    cu[k] = mk[k] ? drh : cc0;
+    cmax = (k == 2) ? cu[k] : mmax(cmax, cu[k]);
+    if (mk[k]) t.mk |= 1u << k;
+  }
+  int icl = 0;
+#pragma unroll
+  for (int k = 2; k <= KX - 3; ++k) {        // Q5: float equality
+    const int cand = (cmax == cu[k]) ? (mk[k] ? k : icl0) : icl0;
+    icl = (k == 2) ? cand : (cand > icl ? cand : icl);
+  }
+  const R pr1 = mmin(p.pmaxcl, R(86.4) * (a.precnv + a.precls));
+  const R cl2 = mmin(R(1.0), cmax * rrcl);
+  const R ccraw = p.wpcl * msqrt(mmax(R(EPS), pr1)) + sq(cl2);
+  R cloudc = mmin(R(1.0), ccraw);
+  if (cloudc != cloudc) cloudc = R(1.0);
+  a.cloudc = cloudc;
+  a.icltop = a.iptop < icl ? a.iptop : icl;
+  a.qcloud = c.Q[KX - 2];
+  const R rgse = u.rgse;
+  const R fsa = rgse * (a.gse - p.gse_s0);
+  const R fstab = mmin(mmax(fsa, R(0.0)), R(1.0));
+  const R clsx = p.clsmax - R(1.2) * cloudc;
+  R clstr = fstab * mmax(clsx, R(0.0));
+  const R clstrl = mmax(clstr, p.clsminl) * a.rh[KX - 1];
+  t.m0 = m0; t.cmax = cmax; t.pr1 = pr1; t.ccraw = ccraw; t.cl2 = cl2;
+  t.fstab_arg = fsa; t.fstab = fstab; t.clsx = clsx; t.clstr0 = clstr;
+  a.clstr = clstr + c.fmask * (clstrl - clstr);
+}
+
+template <typename R, int KX>
+SPD_HD void phase_a(const ColumnIn<R, KX>& c, const SpeedyParams& p, const SpeedyVertical& v,
+                    const Uniforms<R>& u, PhaseA<R, KX>& a) {
+  humidity<R, KX>(c, v, a);
+  ConvTape<R, KX> ct;
+  convection<R, KX>(c, p, v, u, a, ct);
+  condensation<R, KX>(c, u, a);
+  CloudTape<R, KX> cl;
+  clouds<R, KX>(c, p, u, a, cl);
 }
 
 // Cloud top only (for the level set S): the decisions of phase A without the
@@ -529,6 +598,7 @@ struct PhaseB {
   // b2
   R tsw[KX], tlw[KX];
   R du1, dv1, dts1, dqs1;        // surface-flux tendencies at level KX-1
+  R flux_h[2], rlus2;            // shortwave -> longwave and surface -> longwave hand-offs (and tsfc)
   // diagnostics (b2)
   R dfabs_sw[KX], rsds, rsns, ftop_sw, stratc0, stratc1;
   R rlns, ftop_lw, flux_out[4];
@@ -546,360 +616,466 @@ SPD_HD R exp_tr(R x) {
   return mexp(x);
 }
 
+// set_forcing albedos (speedy/forcing.py:26-30)
 template <typename R, int KX>
-SPD_HD void phase_b1(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
-                     const SpeedyVertical& v, const Uniforms<R>& u, PhaseB<R, KX>& b) {
-  // ---- set_forcing albedos (speedy/forcing.py:26-30)
+SPD_HD void albedos(const ColumnIn<R, KX>& c, const SpeedyParams& p, PhaseB<R, KX>& b) {
   b.snowc = mmin(R(1.0), c.snowc_am);
   b.alb_l = c.alb0 + b.snowc * (p.albsn - c.alb0);
   b.alb_s = p.albsea + c.sice * (R(p.albice) - p.albsea);
   b.albsfc = b.alb_s + c.fmask * (b.alb_l - b.alb_s);
-  const R ps = c.ps;
-  // ---- longwave transmissivity (shortwave_radiation.py:177-200)
+}
+
+// Longwave absorptivities of level k per band (shortwave_radiation.py:177-200).
+template <typename R, int KX>
+SPD_HD void lw_absorptivity(int k, const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
+                            const Uniforms<R>& u, R ab[4]) {
+  ab[0] = p.ablwin; ab[1] = u.ablco2; ab[2] = p.ablwv1 * c.Q[k]; ab[3] = p.ablwv2 * c.Q[k];
+  if (k == 0) { ab[2] = R(0.0); ab[3] = R(0.0); }
+  if (k >= 2 && k <= KX - 2) {
+    ab[0] = ab[0] + ((k > a.icltop) ? a.cloudc * p.ablcl1 : a.cloudc * p.ablcl2);
+    const R ac2 = a.cloudc * p.ablcl2;
+    ab[2] = mmax(ab[2], ac2);
+    ab[3] = mmax(ab[3], ac2);
+  }
+}
+
+template <typename R, int KX>
+SPD_HD void lw_transmissivity(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
+                              const SpeedyVertical& v, const Uniforms<R>& u, PhaseB<R, KX>& b) {
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
-    R ab[4] = {p.ablwin, u.ablco2, p.ablwv1 * c.Q[k], p.ablwv2 * c.Q[k]};
-    if (k == 0) { ab[2] = R(0.0); ab[3] = R(0.0); }
-    if (k >= 2 && k <= KX - 2) {
-      ab[0] = ab[0] + ((k > a.icltop) ? a.cloudc * p.ablcl1 : a.cloudc * p.ablcl2);
-      const R ac2 = a.cloudc * p.ablcl2;
-      ab[2] = mmax(ab[2], ac2);
-      ab[3] = mmax(ab[3], ac2);
-    }
+    R ab[4];
+    lw_absorptivity<R, KX>(k, c, a, p, u, ab);
 #pragma unroll
-    for (int bnd = 0; bnd < 4; ++bnd) b.tauL[k][bnd] = exp_tr(-ab[bnd] * ps * v.dhs[k]);
+    for (int bnd = 0; bnd < 4; ++bnd) b.tauL[k][bnd] = exp_tr(-ab[bnd] * c.ps * v.dhs[k]);
   }
-  // ---- longwave down (longwave_radiation.py:13-117)
-  {
-    R Th[KX];
+}
+
+template <typename R, int KX>
+struct LwdTape {
+  R Th[KX], grad[KX], s0, s1;
+  R Fd[KX][4];                    // band fluxes leaving level k downward
+};
+
+// longwave down (longwave_radiation.py:13-117)
+template <typename R, int KX>
+SPD_HD void lw_down(const ColumnIn<R, KX>& c, const SpeedyParams& p, const Uniforms<R>& u, PhaseB<R, KX>& b,
+                    LwdTape<R, KX>& t) {
+  R* Th = t.Th;
 #pragma unroll
-    for (int k = 0; k < KX - 1; ++k) Th[k] = c.T[k] + u.wvi1[k] * (c.T[k + 1] - c.T[k]);
-    const R s0 = R(0.75) * c.T[0] + R(0.25) * Th[0];
-    const R s1 = R(0.50) * c.T[1] + R(0.25) * (Th[0] + Th[1]);
-    R grad[KX];
+  for (int k = 0; k < KX - 1; ++k) Th[k] = c.T[k] + u.wvi1[k] * (c.T[k + 1] - c.T[k]);
+  const R s0 = R(0.75) * c.T[0] + R(0.25) * Th[0];
+  const R s1 = R(0.50) * c.T[1] + R(0.25) * (Th[0] + Th[1]);
+  t.s0 = s0; t.s1 = s1;
+  R* grad = t.grad;
 #pragma unroll
-    for (int k = 2; k < KX - 1; ++k) grad[k] = R(0.5) * mmax(Th[k] - Th[k - 1], R(0.0));
-    grad[KX - 1] = mmax(c.T[KX - 1] - Th[KX - 2], R(0.0));
-    b.st4a0[0] = R(SBC) * pow4(s0); b.st4a0[1] = R(SBC) * pow4(s1);
-    b.st4a1[0] = R(0.0); b.st4a1[1] = R(0.0);
+  for (int k = 2; k < KX - 1; ++k) grad[k] = R(0.5) * mmax(Th[k] - Th[k - 1], R(0.0));
+  grad[KX - 1] = mmax(c.T[KX - 1] - Th[KX - 2], R(0.0));
+  b.st4a0[0] = R(SBC) * pow4(s0); b.st4a0[1] = R(SBC) * pow4(s1);
+  b.st4a1[0] = R(0.0); b.st4a1[1] = R(0.0);
 #pragma unroll
-    for (int k = 2; k < KX; ++k) {
-      const R st3a = R(SBC) * pow3(c.T[k]);
-      b.st4a0[k] = st3a * c.T[k];
-      b.st4a1[k] = R(4.0) * st3a * grad[k];
-    }
-    R Fd[4];
-    {
-      R fb[4];
-      radset4(c.T[0], u.omeps, fb);
-#pragma unroll
-      for (int bnd = 0; bnd < 4; ++bnd) {
-        const R em = R(1.0) - b.tauL[0][bnd];
-        Fd[bnd] = bnd < 2 ? em * (fb[bnd] * (b.st4a0[0] + em * b.st4a1[0])) : R(0.0);
-      }
-    }
-    R Fprev = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
-    b.dfl[0] = -Fprev;
-#pragma unroll
-    for (int k = 1; k < KX; ++k) {
-      R fb[4];
-      radset4(c.T[k], u.omeps, fb);
-#pragma unroll
-      for (int bnd = 0; bnd < 4; ++bnd) {
-        const R em = R(1.0) - b.tauL[k][bnd];
-        Fd[bnd] = b.tauL[k][bnd] * Fd[bnd] + em * (fb[bnd] * (b.st4a0[k] + em * b.st4a1[k]));
-      }
-      const R Fs = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
-      b.dfl[k] = -(Fs - Fprev);
-      Fprev = Fs;
-    }
-    R rlds = ((p.emisfc * Fd[0] + p.emisfc * Fd[1]) + p.emisfc * Fd[2]) + p.emisfc * Fd[3];
-    const R corlw = u.corlw * b.st4a0[KX - 1];
-    b.dfl[KX - 1] = b.dfl[KX - 1] + (-corlw);
-    b.rlds = rlds + corlw;
+  for (int k = 2; k < KX; ++k) {
+    const R st3a = R(SBC) * pow3(c.T[k]);
+    b.st4a0[k] = st3a * c.T[k];
+    b.st4a1[k] = R(4.0) * st3a * grad[k];
   }
-  // ---- vertical diffusion (vertical_diffusion.py:10-135)
+  R Fd[4];
   {
-    const int icnv = KX - a.iptop - 1;
-    constexpr int NL1 = KX - 1;
-    R tt[KX], qt[KX];
+    R fb[4];
+    radset4(c.T[0], u.omeps, fb);
 #pragma unroll
-    for (int k = 0; k < KX; ++k) { tt[k] = R(0.0); qt[k] = R(0.0); }
-    const R dmse = a.se[KX - 1] - a.se[NL1 - 1] + R(ALHC) * (c.Q[KX - 1] - a.qsat[NL1 - 1]);
-    const R drh = a.rh[KX - 1] - a.rh[NL1 - 1];
-    const R fcnv = (icnv > 0 && dmse >= R(0.0)) ? p.redshc : R(1.0);
-    const R rsA = u.rsig[NL1 - 1], rsB = u.rsig[KX - 1];
-    if (dmse >= R(0.0)) {
-      const R fluxse = fcnv * u.fshcse * dmse;
-      tt[NL1 - 1] = fluxse * rsA;
-      tt[KX - 1] = -fluxse * rsB;
-      if (drh >= R(0.0)) {
-        const R fluxq = fcnv * u.fshcq * a.qsat[KX - 1] * drh;
-        qt[NL1 - 1] = fluxq * rsA;
-        qt[KX - 1] = -fluxq * rsB;
-      }
-    } else if (dmse < R(0.0) && drh > u.drh0) {
-      const R fluxq = u.fvdiq2 * a.qsat[NL1 - 1] * drh;
+    for (int bnd = 0; bnd < 4; ++bnd) {
+      const R em = R(1.0) - b.tauL[0][bnd];
+      Fd[bnd] = bnd < 2 ? em * (fb[bnd] * (b.st4a0[0] + em * b.st4a1[0])) : R(0.0);
+      t.Fd[0][bnd] = Fd[bnd];
+    }
+  }
+  R Fprev = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
+  b.dfl[0] = -Fprev;
+#pragma unroll
+  for (int k = 1; k < KX; ++k) {
+    R fb[4];
+    radset4(c.T[k], u.omeps, fb);
+#pragma unroll
+    for (int bnd = 0; bnd < 4; ++bnd) {
+      const R em = R(1.0) - b.tauL[k][bnd];
+      Fd[bnd] = b.tauL[k][bnd] * Fd[bnd] + em * (fb[bnd] * (b.st4a0[k] + em * b.st4a1[k]));
+      t.Fd[k][bnd] = Fd[bnd];
+    }
+    const R Fs = ((Fd[0] + Fd[1]) + Fd[2]) + Fd[3];
+    b.dfl[k] = -(Fs - Fprev);
+    Fprev = Fs;
+  }
+  R rlds = ((p.emisfc * Fd[0] + p.emisfc * Fd[1]) + p.emisfc * Fd[2]) + p.emisfc * Fd[3];
+  const R corlw = u.corlw * b.st4a0[KX - 1];
+  b.dfl[KX - 1] = b.dfl[KX - 1] + (-corlw);
+  b.rlds = rlds + corlw;
+}
+
+// vertical diffusion (vertical_diffusion.py:10-135)
+template <typename R, int KX>
+SPD_HD void vdiff(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p, const Uniforms<R>& u,
+                  PhaseB<R, KX>& b) {
+  const int icnv = KX - a.iptop - 1;
+  constexpr int NL1 = KX - 1;
+  R tt[KX], qt[KX];
+#pragma unroll
+  for (int k = 0; k < KX; ++k) { tt[k] = R(0.0); qt[k] = R(0.0); }
+  const R dmse = a.se[KX - 1] - a.se[NL1 - 1] + R(ALHC) * (c.Q[KX - 1] - a.qsat[NL1 - 1]);
+  const R drh = a.rh[KX - 1] - a.rh[NL1 - 1];
+  const R fcnv = (icnv > 0 && dmse >= R(0.0)) ? p.redshc : R(1.0);
+  const R rsA = u.rsig[NL1 - 1], rsB = u.rsig[KX - 1];
+  if (dmse >= R(0.0)) {
+    const R fluxse = fcnv * u.fshcse * dmse;
+    tt[NL1 - 1] = fluxse * rsA;
+    tt[KX - 1] = -fluxse * rsB;
+    if (drh >= R(0.0)) {
+      const R fluxq = fcnv * u.fshcq * a.qsat[KX - 1] * drh;
       qt[NL1 - 1] = fluxq * rsA;
       qt[KX - 1] = -fluxq * rsB;
     }
-    // moisture diffusion above the PBL (:99-118)
-    R fq[KX];
+  } else if (dmse < R(0.0) && drh > u.drh0) {
+    const R fluxq = u.fvdiq2 * a.qsat[NL1 - 1] * drh;
+    qt[NL1 - 1] = fluxq * rsA;
+    qt[KX - 1] = -fluxq * rsB;
+  }
+  // moisture diffusion above the PBL (:99-118)
+  R fq[KX];
 #pragma unroll
-    for (int k = 2; k < KX - 2; ++k) {
-      const R d = a.rh[k + 1] - a.rh[k];
-      fq[k] = (d >= u.vd_drh0[k] && ((u.vd_on >> k) & 1u)) ? u.vd_fq[k] * a.qsat[k] * d : R(0.0);
+  for (int k = 2; k < KX - 2; ++k) {
+    const R d = a.rh[k + 1] - a.rh[k];
+    fq[k] = (d >= u.vd_drh0[k] && ((u.vd_on >> k) & 1u)) ? u.vd_fq[k] * a.qsat[k] * d : R(0.0);
+  }
+#pragma unroll
+  for (int k = 2; k < KX - 2; ++k) qt[k] = qt[k] + fq[k] * u.rsig[k];
+#pragma unroll
+  for (int k = 2; k < KX - 2; ++k) qt[k + 1] = qt[k + 1] + (-fq[k] * u.rsig[k + 1]);
+  // super-adiabatic damping (:120-131): all local adds, then the cumulative sum
+  R fse[KX];
+#pragma unroll
+  for (int k = 0; k < NL1; ++k) {
+    const R se0 = a.se[k + 1] - p.segrad * (c.PHI[k + 1] - c.PHI[k]);
+    fse[k] = (a.se[k] < se0) ? u.fvdise * (se0 - a.se[k]) : R(0.0);
+    tt[k] = tt[k] + fse[k] * u.rsig[k];
+  }
+  R cum = R(0.0);
+#pragma unroll
+  for (int k = 0; k < NL1; ++k) {
+    cum = (k == 0) ? fse[k] * u.rsig1[k] : cum + fse[k] * u.rsig1[k];
+    tt[k + 1] = tt[k + 1] + (-cum);
+  }
+#pragma unroll
+  for (int k = 0; k < KX; ++k) { b.tvd[k] = tt[k]; b.qvd[k] = qt[k]; }
+}
+
+template <typename R, int KX>
+SPD_HD void phase_b1(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
+                     const SpeedyVertical& v, const Uniforms<R>& u, PhaseB<R, KX>& b) {
+  albedos<R, KX>(c, p, b);
+  lw_transmissivity<R, KX>(c, a, p, v, u, b);
+  LwdTape<R, KX> ld;
+  lw_down<R, KX>(c, p, u, b, ld);
+  vdiff<R, KX>(c, a, p, u, b);
+}
+
+template <typename R, int KX>
+struct SwTape {
+  R a_cl, psaz, acloud;
+  R t22[KX], tau0[KX], tau1[KX], f1[KX], f2[KX], Rf[KX], fu[KX], dfabs[KX];
+};
+
+// shortwave (shortwave_radiation.py:27-217); flux_h is the hand-off to the
+// longwave (Q2).
+template <typename R, int KX, bool DIAG>
+SPD_HD void shortwave(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, uint32_t S, const SpeedyParams& p,
+                      const SpeedyVertical& v, const Uniforms<R>& u, PhaseB<R, KX>& b, R flux_h[4],
+                      SwTape<R, KX>& t) {
+  const R ps = c.ps;
+  const R rps = R(1.0) / ps;
+  const R a_cl = (a.icltop < KX) ? p.albcl * a.cloudc : R(0.0);
+  R* t22 = t.t22;
+#pragma unroll
+  for (int k = 0; k < KX - 1; ++k) t22[k] = ((S >> k) & 1u) ? a_cl : R(0.0);   // Q1
+  t22[KX - 1] = p.albcls * a.clstr;
+  const R psaz = ps * c.zenit;
+  const R acloud = a.cloudc * mmin(p.abscl1 * a.qcloud, p.abscl2);
+  t.a_cl = a_cl; t.psaz = psaz; t.acloud = acloud;
+  R* tau0 = t.tau0;
+  R* tau1 = t.tau1;
+  tau0[0] = exp_tr(-psaz * v.dhs[0] * p.absdry);
+#pragma unroll
+  for (int k = 1; k < KX - 1; ++k) {
+    const R cl = (k >= a.icltop) ? acloud : R(0.0);
+    tau0[k] = exp_tr(-psaz * v.dhs[k] * (u.sw_abs1[k] + p.abswv1 * c.Q[k] + cl));
+  }
+  tau0[KX - 1] = exp_tr(-psaz * v.dhs[KX - 1] * (u.sw_abs1[KX - 1] + p.abswv1 * c.Q[KX - 1]));
+  tau1[0] = R(1.0);
+#pragma unroll
+  for (int k = 1; k < KX; ++k) tau1[k] = exp_tr(-psaz * v.dhs[k] * p.abswv2 * c.Q[k]);
+  // downward
+  R* dfabs = t.dfabs;
+  R* f1 = t.f1;
+  R* f2 = t.f2;
+  R* Rf = t.Rf;
+  const R F0 = c.fsol * R(0.95);
+  dfabs[0] = F0;
+  f1[0] = tau0[0] * (F0 - c.ozupp * ps);
+  dfabs[0] = dfabs[0] + (-f1[0]);
+  dfabs[1] = f1[0];
+  f1[1] = tau0[1] * (f1[0] - c.ozone * ps);
+  dfabs[1] = dfabs[1] + (-f1[1]);
+#pragma unroll
+  for (int k = 2; k < KX; ++k) f1[k] = f1[k - 1] * tau0[k] * (R(1.0) - t22[k]);
+#pragma unroll
+  for (int k = 2; k < KX; ++k) {
+    dfabs[k] = f1[k - 1] * (R(1.0) - t22[k]) * (R(1.0) - tau0[k]);
+    Rf[k] = t22[k] * f1[k - 1];
+  }
+  Rf[0] = t22[0];
+  Rf[1] = t22[1];
+  f2[0] = c.fsol * R(0.05);
+  R carry = f2[0];
+#pragma unroll
+  for (int k = 1; k < KX; ++k) { carry = carry * tau1[k]; f2[k] = carry; }
+#pragma unroll
+  for (int k = 1; k < KX; ++k) dfabs[k] = dfabs[k] + f2[k - 1] * (R(1.0) - tau1[k]);
+  // upward
+  const R rsds = f1[KX - 1] + f2[KX - 1];
+  R U = f1[KX - 1] * b.albsfc;
+  const R rsns = rsds - U;
+  R* fu = t.fu;
+  fu[KX - 1] = U;
+#pragma unroll
+  for (int k = KX - 1; k >= 1; --k) { U = U * tau0[k] + Rf[k]; fu[k - 1] = U; }
+#pragma unroll
+  for (int k = 0; k < KX; ++k) dfabs[k] = dfabs[k] + fu[k] * (R(1.0) - tau0[k]);
+  const R ftoa = tau0[0] * fu[0] + Rf[0];
+  const bool csw = u.compute_sw != 0u;
+#pragma unroll
+  for (int k = 0; k < KX; ++k) b.tsw[k] = csw ? dfabs[k] * v.grdscp[k] * rps : R(0.0);
+  flux_h[0] = ftoa; flux_h[1] = f2[KX - 1]; flux_h[2] = R(0.0); flux_h[3] = R(0.0);   // Q2
+  b.rsds = rsds;
+  if (DIAG) {
+#pragma unroll
+    for (int k = 0; k < KX; ++k) b.dfabs_sw[k] = dfabs[k];
+    b.rsns = rsns;
+    b.ftop_sw = c.fsol - ftoa;
+  }
+}
+
+template <typename R>
+struct SfcTape {
+  R u0, v0, Tb, Qb, phi0, dt1, t1_0a, t1_1a, t2_0, t2_1, t1_0, t1_1, t0, wind, den0, tskin0, dd, dthl, den1;
+  R forog, cdldv, shf0a, q1_0, qsat00, evx, evap0a, tsk3, drls, rlus0a, hfl0a, clamb, qs1p, qs1, dtskin;
+  R ds, dths, den2, q1_1, qsat01;
+  R ustr0, vstr0, shf0, evap0, rlus0, ustr1, vstr1, shf1, evap1, rlus1;
+  R ustr2, vstr2, shf2, evap2, rlus2;
+  bool land, warm;
+};
+
+// surface fluxes (surface_flux.py:15-300).  Returns rlus2 and tsfc through t.
+template <typename R, int KX, bool DIAG>
+SPD_HD void surface(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p, const SpeedyVertical& v,
+                    const Uniforms<R>& u, uint32_t flags, PhaseB<R, KX>& b, R& tsfc, SfcTape<R>& t) {
+  const SpeedyParams& s = p;
+  const R ps = c.ps;
+  const R rps = R(1.0) / ps;
+  const R rlds = b.rlds;
+  const R u1 = c.u1, v1 = c.v1, Tb = c.T[KX - 1], Qb = c.Q[KX - 1];
+  const R u0 = s.fwind0 * u1, v0 = s.fwind0 * v1;
+  t.u0 = u0; t.v0 = v0; t.Tb = Tb; t.Qb = Qb;
+  R t1_0 = R(0.0), t1_1 = R(0.0), t2_1 = R(0.0), q1_1 = R(0.0);
+  R den0 = R(0.0), den2 = R(0.0);
+  R ustr0 = R(0.0), vstr0 = R(0.0), shf0 = R(0.0), evap0 = R(0.0), rlus0 = R(0.0), hfl0 = R(0.0), tskin = R(0.0);
+  R ustr1 = R(0.0), vstr1 = R(0.0);
+  const bool land = flags & SPEEDY_FLAG_LFLUXLAND;
+  t.land = land;
+  if (land) {
+    const R phi0 = c.orog * R(GRAV);
+    const R dt1 = v.wvi[KX - 1][1] * (Tb - c.T[KX - 2]);
+    t1_0 = Tb + dt1;
+    t1_1 = t1_0 - phi0 * dt1 / (R(RGAS288) * v.sigl[KX - 1]);
+    t2_1 = Tb + R(RCP) * c.PHI[KX - 1];
+    const R t2_0 = t2_1 - R(RCP) * phi0;
+    t.phi0 = phi0; t.dt1 = dt1; t.t1_0a = t1_0; t.t1_1a = t1_1; t.t2_0 = t2_0; t.t2_1 = t2_1;
+    t.warm = Tb > c.T[KX - 2];
+    if (Tb > c.T[KX - 2]) {
+      t1_0 = s.ftemp0 * t1_0 + u.gtemp0 * t2_0;
+      t1_1 = s.ftemp0 * t1_1 + u.gtemp0 * t2_1;
+    } else {
+      t1_0 = Tb;
+      t1_1 = Tb;
     }
-#pragma unroll
-    for (int k = 2; k < KX - 2; ++k) qt[k] = qt[k] + fq[k] * u.rsig[k];
-#pragma unroll
-    for (int k = 2; k < KX - 2; ++k) qt[k + 1] = qt[k + 1] + (-fq[k] * u.rsig[k + 1]);
-    // super-adiabatic damping (:120-131): all local adds, then the cumulative sum
-    R fse[KX];
-#pragma unroll
-    for (int k = 0; k < NL1; ++k) {
-      const R se0 = a.se[k + 1] - p.segrad * (c.PHI[k + 1] - c.PHI[k]);
-      fse[k] = (a.se[k] < se0) ? u.fvdise * (se0 - a.se[k]) : R(0.0);
-      tt[k] = tt[k] + fse[k] * u.rsig[k];
+    const R t0 = t1_1 + c.fmask * (t1_0 - t1_1);
+    const R wind = msqrt(u0 * u0 + v0 * v0 + R(s.vgust) * s.vgust);
+    den0 = (R(P0) * ps / (R(RGAS) * t0)) * wind;
+    tskin = c.stl + s.ctday * c.sqcoa * b.rsds * (R(1.0) - b.alb_l) * ps;
+    t.t1_0 = t1_0; t.t1_1 = t1_1; t.t0 = t0; t.wind = wind; t.den0 = den0; t.tskin0 = tskin;
+    const R dd = tskin - t2_0;
+    const R dthl = (tskin > t2_0) ? mmin(R(s.dtheta), dd) : mmax(-R(s.dtheta), u.astab * dd);
+    const R den1 = den0 * (R(1.0) + dthl * u.rdth);
+    const R forog = R(1.0) + u.rhdrag * (R(1.0) - exp_tr(-mmax(c.phis0, R(0.0)) * u.rhdrag));
+    const R cdldv = u.cdl * den0 * forog;
+    t.dd = dd; t.dthl = dthl; t.den1 = den1; t.forog = forog; t.cdldv = cdldv;
+    ustr0 = -cdldv * u1;
+    vstr0 = -cdldv * v1;
+    shf0 = u.chlcp * den1 * (tskin - t1_0);
+    t.shf0a = shf0;
+    R q1_0 = Qb;
+    if (s.fhum0 > R(0.0)) q1_0 = s.fhum0 * (a.rh[KX - 1] * qsat_f(t1_0, ps, R(1.0))) + u.ghum0 * Qb;
+    const R qsat00 = qsat_f(tskin, ps, R(1.0));
+    const R evx = c.soilw * qsat00 - q1_0;
+    evap0 = s.chl * den1 * mmax(R(0.0), evx);
+    const R tsk3 = pow3(tskin);
+    const R drls = R(4.0) * u.esbc * tsk3;
+    rlus0 = u.esbc * tsk3 * tskin;
+    hfl0 = b.rsds * (R(1.0) - b.alb_l) + rlds - (rlus0 + shf0 + (R(ALHC) * evap0));
+    t.q1_0 = q1_0; t.qsat00 = qsat00; t.evx = evx; t.evap0a = evap0; t.tsk3 = tsk3; t.drls = drls;
+    t.rlus0a = rlus0; t.hfl0a = hfl0;
+    if (s.lskineb) {
+      const R clamb = s.clambda + (b.snowc * (R(s.clambsn) - s.clambda));
+      hfl0 = hfl0 - (clamb * (tskin - c.stl));
+      const R qs1p = qsat_f(tskin + R(1.0), ps, R(1.0));
+      const R qs1 = (evap0 > R(0.0)) ? c.soilw * (qs1p - qsat00) : R(0.0);
+      const R dtskin = hfl0 / (clamb + drls + (s.chl * den1 * (R(CP) + (R(ALHC) * qs1))));
+      t.clamb = clamb; t.qs1p = qs1p; t.qs1 = qs1; t.dtskin = dtskin;
+      tskin = tskin + dtskin;
+      shf0 = shf0 + u.chlcp * den1 * dtskin;
+      evap0 = evap0 + s.chl * den1 * qs1 * dtskin;
+      rlus0 = rlus0 + drls * dtskin;
+      hfl0 = clamb * (tskin - c.stl);
     }
-    R cum = R(0.0);
+    const R ds = c.sst - t2_1;
+    const R dths = (c.sst > t2_1) ? mmin(R(s.dtheta), ds) : mmax(-R(s.dtheta), u.astab * ds);
+    den2 = den0 * (R(1.0) + dths * u.rdth);
+    q1_1 = Qb;
+    if (s.fhum0 > R(0.0)) q1_1 = s.fhum0 * (a.rh[KX - 1] * qsat_f(t1_1, ps, R(1.0))) + u.ghum0 * Qb;
+    const R cdsdv = u.cds * den2;
+    ustr1 = -cdsdv * u1;
+    vstr1 = -cdsdv * v1;
+    t.ds = ds; t.dths = dths; t.den2 = den2; t.q1_1 = q1_1;
+  }
+  // sea (:250-261)
+  const R shf1 = u.chscp * den2 * (c.sst - t1_1);
+  const R qsat01 = qsat_f(c.sst, ps, R(1.0));
+  const R evap1 = s.chs * den2 * (qsat01 - q1_1);
+  const R rlus1 = u.esbc * pow4(c.sst);
+  const R hfl1 = b.rsds * (R(1.0) - b.alb_s) + rlds - rlus1 + shf1 + R(ALHC) * evap1;
+  t.qsat01 = qsat01;
+  R ustr2 = R(0.0), vstr2 = R(0.0), shf2 = R(0.0), evap2 = R(0.0), t0o = R(0.0), rlus2 = R(0.0);
+  tsfc = R(0.0);
+  R tskin_o = R(0.0);
+  if (land) {
+    const R fm = c.fmask;
+    ustr2 = ustr1 + fm * (ustr0 - ustr1);
+    vstr2 = vstr1 + fm * (vstr0 - vstr1);
+    shf2 = shf1 + fm * (shf0 - shf1);
+    evap2 = evap1 + fm * (evap0 - evap1);
+    rlus2 = rlus1 + fm * (rlus0 - rlus1);
+    t0o = t1_1 + fm * (t1_0 - t1_1);
+    tsfc = c.sst + fm * (c.stl - c.sst);
+    tskin_o = c.sst + fm * (tskin - c.sst);
+  }
+  t.ustr0 = ustr0; t.vstr0 = vstr0; t.shf0 = shf0; t.evap0 = evap0; t.rlus0 = rlus0;
+  t.ustr1 = ustr1; t.vstr1 = vstr1; t.shf1 = shf1; t.evap1 = evap1; t.rlus1 = rlus1;
+  t.ustr2 = ustr2; t.vstr2 = vstr2; t.shf2 = shf2; t.evap2 = evap2; t.rlus2 = rlus2;
+  b.du1 = ustr2 * rps * v.grdsig[KX - 1];
+  b.dv1 = vstr2 * rps * v.grdsig[KX - 1];
+  b.dts1 = shf2 * rps * v.grdscp[KX - 1];
+  b.dqs1 = evap2 * rps * v.grdsig[KX - 1];
+  if (DIAG) {
+    b.ustr[0] = ustr0; b.ustr[1] = ustr1; b.ustr[2] = ustr2;
+    b.vstr[0] = vstr0; b.vstr[1] = vstr1; b.vstr[2] = vstr2;
+    b.shf[0] = shf0; b.shf[1] = shf1; b.shf[2] = shf2;
+    b.evap[0] = evap0; b.evap[1] = evap1; b.evap[2] = evap2;
+    b.rlus[0] = rlus0; b.rlus[1] = rlus1; b.rlus[2] = rlus2;
+    b.hfluxn[0] = hfl0; b.hfluxn[1] = hfl1;
+    b.tsfc = tsfc; b.tskin = tskin_o; b.u0 = u0; b.v0 = v0; b.t0 = t0o;
+  }
+}
+
+template <typename R, int KX>
+struct LwuTape {
+  R Fu[KX + 1][4];                // Fu[k]: band fluxes leaving level k upward (Fu[KX]: from the surface)
+  R ft[2], dfl[KX];
+};
+
+// longwave up (longwave_radiation.py:119-202); dfl enters holding the
+// longwave-down flux divergence.
+template <typename R, int KX, bool DIAG>
+SPD_HD void lw_up(const ColumnIn<R, KX>& c, const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u,
+                  PhaseB<R, KX>& b, R rlus2, R tsfc, const R flux_h[4], LwuTape<R, KX>& t) {
+  const R ps = c.ps;
+  const R rps = R(1.0) / ps;
+  const R stratc0 = c.stratz * ps, stratc1 = u.eps1 * ps;
+  const R rlds = b.rlds;
+  R fbt[4];
+  radset4(tsfc, u.omeps, fbt);
+  R Fu[4];
 #pragma unroll
-    for (int k = 0; k < NL1; ++k) {
-      cum = (k == 0) ? fse[k] * u.rsig1[k] : cum + fse[k] * u.rsig1[k];
-      tt[k + 1] = tt[k + 1] + (-cum);
+  for (int bnd = 0; bnd < 4; ++bnd) { Fu[bnd] = fbt[bnd] * rlus2 + u.refsfc * flux_h[bnd]; t.Fu[KX][bnd] = Fu[bnd]; }
+  R dfl[KX];
+#pragma unroll
+  for (int k = 0; k < KX; ++k) dfl[k] = b.dfl[k];
+  dfl[KX - 1] = dfl[KX - 1] + p.epslw * rlus2;
+  R Gnext = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
+#pragma unroll
+  for (int k = KX - 1; k >= 1; --k) {
+    R fb[4];
+    radset4(c.T[k], u.omeps, fb);
+#pragma unroll
+    for (int bnd = 0; bnd < 4; ++bnd) {
+      const R em = R(1.0) - b.tauL[k][bnd];
+      Fu[bnd] = b.tauL[k][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[k] - em * b.st4a1[k]));
+      t.Fu[k][bnd] = Fu[bnd];
     }
+    const R Gk = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
+    dfl[k] = dfl[k] + (Gnext - Gk);
+    Gnext = Gk;
+  }
+  R ft[4];
+  {
+    R fb[4];
+    radset4(c.T[0], u.omeps, fb);
 #pragma unroll
-    for (int k = 0; k < KX; ++k) { b.tvd[k] = tt[k]; b.qvd[k] = qt[k]; }
+    for (int bnd = 0; bnd < 2; ++bnd) {
+      const R em = R(1.0) - b.tauL[0][bnd];
+      ft[bnd] = b.tauL[0][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[0] - em * b.st4a1[0]));
+    }
+    ft[2] = Fu[2];
+    ft[3] = Fu[3];
+  }
+  t.ft[0] = ft[0]; t.ft[1] = ft[1];
+  dfl[0] = dfl[0] + ((Fu[0] - ft[0]) + (Fu[1] - ft[1]));
+  const R corlw1 = v.dhs[0] * stratc1 * b.st4a0[0] + stratc0;
+  const R corlw2 = v.dhs[1] * stratc1 * b.st4a0[1];
+  dfl[0] = dfl[0] + (-corlw1);
+  dfl[1] = dfl[1] + (-corlw2);
+#pragma unroll
+  for (int k = 0; k < KX; ++k) { b.tlw[k] = dfl[k] * v.grdscp[k] * rps; t.dfl[k] = dfl[k]; }
+  if (DIAG) {
+    b.ftop_lw = (corlw1 + corlw2) + (((ft[0] + ft[1]) + ft[2]) + ft[3]);
+#pragma unroll
+    for (int bnd = 0; bnd < 4; ++bnd) b.flux_out[bnd] = ft[bnd];
+    b.rlns = rlus2 - rlds;
+    b.stratc0 = stratc0; b.stratc1 = stratc1;
+#pragma unroll
+    for (int k = 0; k < KX; ++k) b.dfl[k] = dfl[k];
   }
 }
 
 template <typename R, int KX, bool DIAG>
 SPD_HD void phase_b2(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, uint32_t S, const SpeedyParams& p,
                      const SpeedyVertical& v, const Uniforms<R>& u, uint32_t flags, PhaseB<R, KX>& b) {
-  const R ps = c.ps;
-  const R rps = R(1.0) / ps;
-  // ---- shortwave (shortwave_radiation.py:27-217)
   R flux_h[4];
-  {
-    const R a_cl = (a.icltop < KX) ? p.albcl * a.cloudc : R(0.0);
-    R t22[KX];
-#pragma unroll
-    for (int k = 0; k < KX - 1; ++k) t22[k] = ((S >> k) & 1u) ? a_cl : R(0.0);   // Q1
-    t22[KX - 1] = p.albcls * a.clstr;
-    const R psaz = ps * c.zenit;
-    const R acloud = a.cloudc * mmin(p.abscl1 * a.qcloud, p.abscl2);
-    R tau0[KX], tau1[KX];
-    tau0[0] = exp_tr(-psaz * v.dhs[0] * p.absdry);
-#pragma unroll
-    for (int k = 1; k < KX - 1; ++k) {
-      const R cl = (k >= a.icltop) ? acloud : R(0.0);
-      tau0[k] = exp_tr(-psaz * v.dhs[k] * (u.sw_abs1[k] + p.abswv1 * c.Q[k] + cl));
-    }
-    tau0[KX - 1] = exp_tr(-psaz * v.dhs[KX - 1] * (u.sw_abs1[KX - 1] + p.abswv1 * c.Q[KX - 1]));
-#pragma unroll
-    for (int k = 1; k < KX; ++k) tau1[k] = exp_tr(-psaz * v.dhs[k] * p.abswv2 * c.Q[k]);
-    // downward
-    R dfabs[KX], f1[KX], f2[KX], Rf[KX];
-    const R F0 = c.fsol * R(0.95);
-    dfabs[0] = F0;
-    f1[0] = tau0[0] * (F0 - c.ozupp * ps);
-    dfabs[0] = dfabs[0] + (-f1[0]);
-    dfabs[1] = f1[0];
-    f1[1] = tau0[1] * (f1[0] - c.ozone * ps);
-    dfabs[1] = dfabs[1] + (-f1[1]);
-#pragma unroll
-    for (int k = 2; k < KX; ++k) f1[k] = f1[k - 1] * tau0[k] * (R(1.0) - t22[k]);
-#pragma unroll
-    for (int k = 2; k < KX; ++k) {
-      dfabs[k] = f1[k - 1] * (R(1.0) - t22[k]) * (R(1.0) - tau0[k]);
-      Rf[k] = t22[k] * f1[k - 1];
-    }
-    Rf[0] = t22[0];
-    Rf[1] = t22[1];
-    f2[0] = c.fsol * R(0.05);
-    R carry = f2[0];
-#pragma unroll
-    for (int k = 1; k < KX; ++k) { carry = carry * tau1[k]; f2[k] = carry; }
-#pragma unroll
-    for (int k = 1; k < KX; ++k) dfabs[k] = dfabs[k] + f2[k - 1] * (R(1.0) - tau1[k]);
-    // upward
-    const R rsds = f1[KX - 1] + f2[KX - 1];
-    R U = f1[KX - 1] * b.albsfc;
-    const R rsns = rsds - U;
-    R fu[KX];
-    fu[KX - 1] = U;
-#pragma unroll
-    for (int k = KX - 1; k >= 1; --k) { U = U * tau0[k] + Rf[k]; fu[k - 1] = U; }
-#pragma unroll
-    for (int k = 0; k < KX; ++k) dfabs[k] = dfabs[k] + fu[k] * (R(1.0) - tau0[k]);
-    const R ftoa = tau0[0] * fu[0] + Rf[0];
-    const bool csw = u.compute_sw != 0u;
-#pragma unroll
-    for (int k = 0; k < KX; ++k) b.tsw[k] = csw ? dfabs[k] * v.grdscp[k] * rps : R(0.0);
-    flux_h[0] = ftoa; flux_h[1] = f2[KX - 1]; flux_h[2] = R(0.0); flux_h[3] = R(0.0);   // Q2
-    b.rsds = rsds;
-    if (DIAG) {
-#pragma unroll
-      for (int k = 0; k < KX; ++k) b.dfabs_sw[k] = dfabs[k];
-      b.rsns = rsns;
-      b.ftop_sw = c.fsol - ftoa;
-    }
-  }
-  const R stratc0 = c.stratz * ps, stratc1 = u.eps1 * ps;
-  const R rlds = b.rlds;
-  // ---- surface fluxes (surface_flux.py:15-300)
-  R rlus2, tsfc;
-  {
-    const SpeedyParams& s = p;
-    const R u1 = c.u1, v1 = c.v1, Tb = c.T[KX - 1], Qb = c.Q[KX - 1];
-    const R u0 = s.fwind0 * u1, v0 = s.fwind0 * v1;
-    R t1_0 = R(0.0), t1_1 = R(0.0), t2_1 = R(0.0), q1_1 = R(0.0);
-    R den0 = R(0.0), den2 = R(0.0);
-    R ustr0 = R(0.0), vstr0 = R(0.0), shf0 = R(0.0), evap0 = R(0.0), rlus0 = R(0.0), hfl0 = R(0.0), tskin = R(0.0);
-    R ustr1 = R(0.0), vstr1 = R(0.0);
-    const bool land = flags & SPEEDY_FLAG_LFLUXLAND;
-    if (land) {
-      const R phi0 = c.orog * R(GRAV);
-      const R dt1 = v.wvi[KX - 1][1] * (Tb - c.T[KX - 2]);
-      t1_0 = Tb + dt1;
-      t1_1 = t1_0 - phi0 * dt1 / (R(RGAS288) * v.sigl[KX - 1]);
-      t2_1 = Tb + R(RCP) * c.PHI[KX - 1];
-      const R t2_0 = t2_1 - R(RCP) * phi0;
-      if (Tb > c.T[KX - 2]) {
-        t1_0 = s.ftemp0 * t1_0 + u.gtemp0 * t2_0;
-        t1_1 = s.ftemp0 * t1_1 + u.gtemp0 * t2_1;
-      } else {
-        t1_0 = Tb;
-        t1_1 = Tb;
-      }
-      const R t0 = t1_1 + c.fmask * (t1_0 - t1_1);
-      den0 = (R(P0) * ps / (R(RGAS) * t0)) * msqrt(u0 * u0 + v0 * v0 + R(s.vgust) * s.vgust);
-      tskin = c.stl + s.ctday * c.sqcoa * b.rsds * (R(1.0) - b.alb_l) * ps;
-      const R dd = tskin - t2_0;
-      const R dthl = (tskin > t2_0) ? mmin(s.dtheta, dd) : mmax(-s.dtheta, u.astab * dd);
-      const R den1 = den0 * (R(1.0) + dthl * u.rdth);
-      const R forog = R(1.0) + u.rhdrag * (R(1.0) - exp_tr(-mmax(c.phis0, R(0.0)) * u.rhdrag));
-      const R cdldv = u.cdl * den0 * forog;
-      ustr0 = -cdldv * u1;
-      vstr0 = -cdldv * v1;
-      shf0 = u.chlcp * den1 * (tskin - t1_0);
-      R q1_0 = Qb;
-      if (s.fhum0 > R(0.0)) q1_0 = s.fhum0 * (a.rh[KX - 1] * qsat_f(t1_0, ps, R(1.0))) + u.ghum0 * Qb;
-      const R qsat00 = qsat_f(tskin, ps, R(1.0));
-      evap0 = s.chl * den1 * mmax(R(0.0), c.soilw * qsat00 - q1_0);
-      const R tsk3 = pow3(tskin);
-      const R drls = R(4.0) * u.esbc * tsk3;
-      rlus0 = u.esbc * tsk3 * tskin;
-      hfl0 = b.rsds * (R(1.0) - b.alb_l) + rlds - (rlus0 + shf0 + (R(ALHC) * evap0));
-      if (s.lskineb) {
-        const R clamb = s.clambda + (b.snowc * (R(s.clambsn) - s.clambda));
-        hfl0 = hfl0 - (clamb * (tskin - c.stl));
-        R qs1 = qsat_f(tskin + R(1.0), ps, R(1.0));
-        qs1 = (evap0 > R(0.0)) ? c.soilw * (qs1 - qsat00) : R(0.0);
-        const R dtskin = hfl0 / (clamb + drls + (s.chl * den1 * (R(CP) + (R(ALHC) * qs1))));
-        tskin = tskin + dtskin;
-        shf0 = shf0 + u.chlcp * den1 * dtskin;
-        evap0 = evap0 + s.chl * den1 * qs1 * dtskin;
-        rlus0 = rlus0 + drls * dtskin;
-        hfl0 = clamb * (tskin - c.stl);
-      }
-      const R ds = c.sst - t2_1;
-      const R dths = (c.sst > t2_1) ? mmin(s.dtheta, ds) : mmax(-s.dtheta, u.astab * ds);
-      den2 = den0 * (R(1.0) + dths * u.rdth);
-      q1_1 = Qb;
-      if (s.fhum0 > R(0.0)) q1_1 = s.fhum0 * (a.rh[KX - 1] * qsat_f(t1_1, ps, R(1.0))) + u.ghum0 * Qb;
-      const R cdsdv = u.cds * den2;
-      ustr1 = -cdsdv * u1;
-      vstr1 = -cdsdv * v1;
-    }
-    // sea (:250-261)
-    const R shf1 = u.chscp * den2 * (c.sst - t1_1);
-    const R qsat01 = qsat_f(c.sst, ps, R(1.0));
-    const R evap1 = s.chs * den2 * (qsat01 - q1_1);
-    const R rlus1 = u.esbc * pow4(c.sst);
-    const R hfl1 = b.rsds * (R(1.0) - b.alb_s) + rlds - rlus1 + shf1 + R(ALHC) * evap1;
-    R ustr2 = R(0.0), vstr2 = R(0.0), shf2 = R(0.0), evap2 = R(0.0), t0o = R(0.0);
-    rlus2 = R(0.0); tsfc = R(0.0);
-    R tskin_o = R(0.0);
-    if (land) {
-      const R fm = c.fmask;
-      ustr2 = ustr1 + fm * (ustr0 - ustr1);
-      vstr2 = vstr1 + fm * (vstr0 - vstr1);
-      shf2 = shf1 + fm * (shf0 - shf1);
-      evap2 = evap1 + fm * (evap0 - evap1);
-      rlus2 = rlus1 + fm * (rlus0 - rlus1);
-      t0o = t1_1 + fm * (t1_0 - t1_1);
-      tsfc = c.sst + fm * (c.stl - c.sst);
-      tskin_o = c.sst + fm * (tskin - c.sst);
-    }
-    b.du1 = ustr2 * rps * v.grdsig[KX - 1];
-    b.dv1 = vstr2 * rps * v.grdsig[KX - 1];
-    b.dts1 = shf2 * rps * v.grdscp[KX - 1];
-    b.dqs1 = evap2 * rps * v.grdsig[KX - 1];
-    if (DIAG) {
-      b.ustr[0] = ustr0; b.ustr[1] = ustr1; b.ustr[2] = ustr2;
-      b.vstr[0] = vstr0; b.vstr[1] = vstr1; b.vstr[2] = vstr2;
-      b.shf[0] = shf0; b.shf[1] = shf1; b.shf[2] = shf2;
-      b.evap[0] = evap0; b.evap[1] = evap1; b.evap[2] = evap2;
-      b.rlus[0] = rlus0; b.rlus[1] = rlus1; b.rlus[2] = rlus2;
-      b.hfluxn[0] = hfl0; b.hfluxn[1] = hfl1;
-      b.tsfc = tsfc; b.tskin = tskin_o; b.u0 = u0; b.v0 = v0; b.t0 = t0o;
-    }
-  }
-  // ---- longwave up (longwave_radiation.py:119-202)
-  {
-    R fbt[4];
-    radset4(tsfc, u.omeps, fbt);
-    R Fu[4];
-#pragma unroll
-    for (int bnd = 0; bnd < 4; ++bnd) Fu[bnd] = fbt[bnd] * rlus2 + u.refsfc * flux_h[bnd];
-    R dfl[KX];
-#pragma unroll
-    for (int k = 0; k < KX; ++k) dfl[k] = b.dfl[k];
-    dfl[KX - 1] = dfl[KX - 1] + p.epslw * rlus2;
-    R Gnext = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
-#pragma unroll
-    for (int k = KX - 1; k >= 1; --k) {
-      R fb[4];
-      radset4(c.T[k], u.omeps, fb);
-#pragma unroll
-      for (int bnd = 0; bnd < 4; ++bnd) {
-        const R em = R(1.0) - b.tauL[k][bnd];
-        Fu[bnd] = b.tauL[k][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[k] - em * b.st4a1[k]));
-      }
-      const R Gk = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
-      dfl[k] = dfl[k] + (Gnext - Gk);
-      Gnext = Gk;
-    }
-    R ft[4];
-    {
-      R fb[4];
-      radset4(c.T[0], u.omeps, fb);
-#pragma unroll
-      for (int bnd = 0; bnd < 2; ++bnd) {
-        const R em = R(1.0) - b.tauL[0][bnd];
-        ft[bnd] = b.tauL[0][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[0] - em * b.st4a1[0]));
-      }
-      ft[2] = Fu[2];
-      ft[3] = Fu[3];
-    }
-    dfl[0] = dfl[0] + ((Fu[0] - ft[0]) + (Fu[1] - ft[1]));
-    const R corlw1 = v.dhs[0] * stratc1 * b.st4a0[0] + stratc0;
-    const R corlw2 = v.dhs[1] * stratc1 * b.st4a0[1];
-    dfl[0] = dfl[0] + (-corlw1);
-    dfl[1] = dfl[1] + (-corlw2);
-#pragma unroll
-    for (int k = 0; k < KX; ++k) b.tlw[k] = dfl[k] * v.grdscp[k] * rps;
-    if (DIAG) {
-      b.ftop_lw = (corlw1 + corlw2) + (((ft[0] + ft[1]) + ft[2]) + ft[3]);
-#pragma unroll
-      for (int bnd = 0; bnd < 4; ++bnd) b.flux_out[bnd] = ft[bnd];
-      b.rlns = rlus2 - rlds;
-      b.stratc0 = stratc0; b.stratc1 = stratc1;
-#pragma unroll
-      for (int k = 0; k < KX; ++k) b.dfl[k] = dfl[k];
-    }
-  }
+  SwTape<R, KX> sw;
+  shortwave<R, KX, DIAG>(c, a, S, p, v, u, b, flux_h, sw);
+  R tsfc;
+  SfcTape<R> sf;
+  surface<R, KX, DIAG>(c, a, p, v, u, flags, b, tsfc, sf);
+  b.flux_h[0] = flux_h[0]; b.flux_h[1] = flux_h[1]; b.rlus2 = sf.rlus2; b.tsfc = tsfc;
+  LwuTape<R, KX> lu;
+  lw_up<R, KX, DIAG>(c, p, v, u, b, sf.rlus2, tsfc, flux_h, lu);
 }
 
 }  // namespace speedy

@@ -126,3 +126,31 @@ def oracle64_same_constants(state, forcing, geometry, date, params=None, member=
 def run_oracle64_same_constants(case, params=None, level_set=None, term_mask=so.ALL_TERMS_MASK, member=None):
     st, fo, g, d, p = oracle64_same_constants(*case, params=params, member=member)
     return so.compute_tendencies(st, fo, g, d, p, level_set=level_set, term_mask=term_mask, return_level_set=True)
+
+
+def run_host_vjp(case, params, cot, level_set=-1, dtype=np.float64):
+    """Forward + VJP of the host build of the column routine (speedy_adjoint.hpp).
+
+    cot: (ct_du, ct_dv, ct_dt, ct_dq), each (kx, ix, il).  Returns (tendencies,
+    grads) with grads = dict(u, v, t, q: (kx, ix, il); ps: (ix, il); prm: (ix, il, 5))
+    where prm holds each column's contribution to the five parameter gradients."""
+    import jcm_amd._native as nat
+    state, forcing, geometry, date = case
+    lib = build_host_column()
+    kx, ix, il = geometry.nodal_shape
+    cv = lambda a: np.ascontiguousarray(a, dtype)
+    ins = [cv(state.u_wind), cv(state.v_wind), cv(state.temperature), cv(state.specific_humidity),
+           cv(state.geopotential), cv(state.normalized_surface_pressure), cv(forcing.alb0), cv(forcing.sice_am),
+           cv(forcing.snowc_am), cv(forcing.soilw_am), cv(forcing.stl_am), cv(forcing.sea_surface_temperature),
+           cv(geometry.fmask), cv(geometry.orog), cv(geometry.phis0), cv(geometry.sia), cv(geometry.coa)]
+    cts = [cv(x) for x in cot]
+    outs = [np.zeros((kx, ix, il), dtype) for _ in range(4)]
+    gs = [np.zeros((kx, ix, il), dtype) for _ in range(4)] + [np.zeros((ix, il), dtype),
+                                                              np.zeros((ix, il, 5), dtype)]
+    P = lambda arrs: (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    flags = nat.FLAG_LFLUXLAND if forcing.lfluxland else 0
+    fn = lib.column_host_vjp if np.dtype(dtype) == np.float32 else lib.column_host_vjp64
+    rc = fn(kx, P(ins), P(cts), P(outs), P(gs), ctypes.byref(geometry.to_abi()), ctypes.byref(params.to_abi()),
+            ctypes.byref(date.to_abi()), ix, il, ctypes.c_uint32(flags), ctypes.c_int32(level_set))
+    assert rc == 0
+    return outs, dict(u=gs[0], v=gs[1], t=gs[2], q=gs[3], ps=gs[4], prm=gs[5])

@@ -1,0 +1,165 @@
+"""The hand-written adjoint (jax-gcm_amd/csrc/speedy_adjoint.hpp), built for the host,
+against finite differences.
+
+The reference's gradient is jax.vjp of the JAX physics, which cannot run here (no
+jax).  It is pinned instead by:
+  * central differences of the float64 build of the same column routine, which is
+    itself pinned to the float64 oracle at 1e-12 (test_host_column), per input
+    field and level, column by column, with the cloud-top level set held fixed
+    (it is a discrete decision, constant under differentiation);
+  * central differences of the float64 oracle itself along a random direction
+    (dot-product test), so the gradient is checked against the oracle directly;
+  * the parameter gradients against central differences of the parameters.
+Columns where the forward has a kink within the step (a branch flips: two step
+sizes disagree) are excluded and counted; the bound is 1e-5 of the field's
+largest derivative (measured: <= 1e-6).
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+from jcm_amd import Parameters, synthetic
+from jcm_amd.params import VJP_PARAMS
+from oracle import speedy_oracle as so
+from test_host_column import CASES, make
+
+FD_TOL = 1e-5
+FIELD_NAMES = dict(t="temperature", q="specific_humidity", u="u_wind", v="v_wind")
+
+
+def _setup(name, seed=5):
+    params = synthetic.draw_vjp_parameters(2) if CASES[name]["wet"] else Parameters.default()
+    case = make(name)
+    kx, ix, il = case[2].nodal_shape
+    rng = np.random.default_rng(seed)
+    cot = [rng.standard_normal((kx, ix, il)) for _ in range(4)]
+    _, diag, S = H.run_host_column(case, params, dtype=np.float64)
+    _, g = H.run_host_vjp(case, params, cot, level_set=S)
+    return params, case, cot, S, g
+
+
+def _J(cot, tend):
+    """Per-column cotangent-weighted tendency sum_k ct . F."""
+    return sum((cot[i] * tend[i]).sum(axis=0) for i in range(4))
+
+
+def _perturbed(case, field, k, h):
+    st, fo, ge, da = case
+    if field == "ps":
+        return (st.copy(normalized_surface_pressure=st.normalized_surface_pressure.astype(np.float64) + h), fo, ge, da)
+    x = getattr(st, FIELD_NAMES[field]).astype(np.float64).copy()
+    x[k] += h
+    return (st.copy(**{FIELD_NAMES[field]: x}), fo, ge, da)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_state_gradients_match_central_differences(name):
+    params, case, cot, S, g = _setup(name)
+    kx = case[2].nodal_shape[0]
+
+    def fd(field, k, h):
+        tp, _, _ = H.run_host_column(_perturbed(case, field, k, h), params, dtype=np.float64, level_set=S)
+        tm, _, _ = H.run_host_column(_perturbed(case, field, k, -h), params, dtype=np.float64, level_set=S)
+        return (_J(cot, tp) - _J(cot, tm)) / (2 * h)
+
+    kinks = 0
+    for field, levels, h in (("t", range(kx), 1e-4), ("q", range(kx), 1e-6), ("u", [kx - 1], 1e-4),
+                             ("v", [kx - 1], 1e-4), ("ps", [0], 1e-7)):
+        for k in levels:
+            f1, f2 = fd(field, k, h), fd(field, k, h / 8)
+            smooth = np.abs(f1 - f2) <= 1e-6 * (np.abs(f1) + np.abs(f1).max())
+            kinks += int((~smooth).sum())
+            grad = g["ps"] if field == "ps" else g[field][k]
+            scale = np.abs(f2).max()
+            if scale == 0:
+                assert np.all(grad == 0), (field, k)
+                continue
+            err = np.where(smooth, np.abs(grad - f2), 0.0).max() / scale
+            assert err <= FD_TOL, f"{field}[{k}]: {err:.2e}"
+    ncol = case[2].ncol
+    assert kinks <= 0.01 * ncol, kinks
+    # u, v enter only at the lowest level (surface stress)
+    assert np.all(g["u"][:-1] == 0) and np.all(g["v"][:-1] == 0)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_parameter_gradients_match_central_differences(name):
+    params, case, cot, S, g = _setup(name)
+    for i, (grp, nm) in enumerate(VJP_PARAMS):
+        v0 = float(np.float32(getattr(getattr(params, grp), nm)))
+
+        def fd(hrel):
+            # the library takes float32 parameters: difference by the values actually used
+            vp, vm = float(np.float32(v0 * (1 + hrel))), float(np.float32(v0 * (1 - hrel)))
+            tp, _, _ = H.run_host_column(case, params.replace(**{grp: {nm: vp}}), dtype=np.float64, level_set=S)
+            tm, _, _ = H.run_host_column(case, params.replace(**{grp: {nm: vm}}), dtype=np.float64, level_set=S)
+            return (_J(cot, tp) - _J(cot, tm)) / (vp - vm)
+
+        f1, f2 = fd(1e-4), fd(2.5e-5)
+        smooth = np.abs(f1 - f2) <= 1e-5 * (np.abs(f1) + np.abs(f1).max() + 1e-30)
+        per_col = g["prm"][..., i]
+        scale = np.abs(f2).max()
+        if scale == 0:
+            assert np.all(per_col == 0), nm
+            continue
+        err = np.where(smooth, np.abs(per_col - f2), 0.0).max() / scale
+        assert err <= FD_TOL, f"{nm}: {err:.2e}"
+        assert (~smooth).sum() <= 0.01 * case[2].ncol
+
+
+@pytest.mark.parametrize("name", ["T21_wet", "T21_wet_kx16", "T31_aqua"])
+def test_gradient_against_float64_oracle_directional_difference(name):
+    """<ct, (F(x + h d) - F(x - h d)) / 2h> of the float64 oracle equals <g, d> for a
+    random direction d in (T, q, u, v, ps) and in the five parameters."""
+    params, case, cot, S, g = _setup(name)
+    st, fo, ge, da = case
+    kx, ix, il = ge.nodal_shape
+    rng = np.random.default_rng(11)
+    d = dict(t=rng.standard_normal((kx, ix, il)), q=rng.standard_normal((kx, ix, il)) * 1e-2,
+             u=rng.standard_normal((kx, ix, il)), v=rng.standard_normal((kx, ix, il)),
+             ps=rng.standard_normal((ix, il)) * 1e-3)
+    lin = sum(float(np.sum(g[k] * d[k])) for k in d)
+    h = 1e-6
+
+    def shifted(sgn):
+        s = st.copy(temperature=st.temperature.astype(np.float64) + sgn * h * d["t"],
+                    specific_humidity=st.specific_humidity.astype(np.float64) + sgn * h * d["q"],
+                    u_wind=st.u_wind.astype(np.float64) + sgn * h * d["u"],
+                    v_wind=st.v_wind.astype(np.float64) + sgn * h * d["v"],
+                    normalized_surface_pressure=st.normalized_surface_pressure.astype(np.float64) + sgn * h * d["ps"])
+        t, _, _ = H.run_oracle64_same_constants((s, fo, ge, da), params, level_set=S)
+        return [t.u_wind, t.v_wind, t.temperature, t.specific_humidity]
+
+    fd = float(np.sum(_J(cot, shifted(1)) - _J(cot, shifted(-1)))) / (2 * h)
+    assert abs(lin - fd) <= 1e-6 * abs(fd), (lin, fd)
+
+
+def test_vjp_finite_at_unit_cotangents():
+    """The reference's VJP tests call jax.vjp with all-ones cotangents and require no
+    NaN (e.g. convection_test.py, shortwave_radiation_test.py); same here, float32."""
+    params, case = Parameters.default(), make("T21_wet")
+    kx, ix, il = case[2].nodal_shape
+    ones = [np.ones((kx, ix, il), np.float32) for _ in range(4)]
+    _, g = H.run_host_vjp(case, params, ones, dtype=np.float32)
+    for k, v in g.items():
+        assert np.all(np.isfinite(v)), k
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_float32_adjoint_close_to_float64(name):
+    """The float32 adjoint (the GPU's arithmetic) against the float64 one: rel-inf
+    per gradient field <= 1e-4 (measured <= 5e-5: cancellation in the convective
+    flux differences and the level-0 shortwave absorption)."""
+    params = synthetic.draw_vjp_parameters(2) if CASES[name]["wet"] else Parameters.default()
+    case = make(name)
+    kx, ix, il = case[2].nodal_shape
+    rng = np.random.default_rng(5)
+    cot = [rng.standard_normal((kx, ix, il)).astype(np.float32) for _ in range(4)]
+    _, _, S = H.run_host_column(case, params)
+    _, g32 = H.run_host_vjp(case, params, cot, level_set=S, dtype=np.float32)
+    _, g64 = H.run_host_vjp(case, params, cot, level_set=S)
+    for k in ("t", "q", "u", "v", "ps"):
+        assert H.rel_inf(g32[k], g64[k]) <= 1e-4, k
+    for i in range(5):
+        a, b = float(g32["prm"][..., i].astype(np.float64).sum()), float(g64["prm"][..., i].sum())
+        assert abs(a - b) <= 1e-4 * max(abs(b), 1e-30) or b == 0, i
