@@ -147,9 +147,16 @@ int speedy_fwd(const SpeedyFields *fields, const SpeedyVertical *vert, const Spe
                uint32_t term_mask, int32_t level_set, void *workspace, size_t workspace_bytes,
                void *stream);
 
-/* Fused forward + VJP: the forward tendencies (as speedy_fwd) and the
- * cotangent-weighted gradients w.r.t. T, q, u, v, ps and five parameters.
- * Replaces jax.vjp of compute_tendencies (speedy_physics.py:88-127). */
+/* Fused forward + VJP: the forward tendencies (as speedy_fwd with every term)
+ * and the cotangent-weighted gradients w.r.t. T, q, u, v, ps and five
+ * parameters.  Replaces jax.vjp of compute_tendencies (speedy_physics.py:88-127)
+ * with respect to PhysicsState (physics_interface.py:22-29) and Parameters
+ * (params.py:22,24,60 and the two library multipliers).  Discrete decisions
+ * (convective and cloud tops, the level set S) are held fixed, as under JAX
+ * differentiation.  g_u and g_v are zero below level kx-1 (only the surface
+ * stress depends on the winds).  fields->diag must be NULL.  The parameter
+ * gradients are summed over each member's columns in a fixed order
+ * (deterministic), by a second small kernel on the same stream. */
 int speedy_fwd_vjp(const SpeedyFields *fields, const SpeedyAdjoint *adj, const SpeedyVertical *vert,
                    const SpeedyParams *params, const SpeedyDate *date, int32_t nlon, int32_t nlat,
                    int32_t nmember, uint32_t flags, int32_t level_set, void *workspace,
@@ -163,8 +170,9 @@ int speedy_read_level_sets(const void *workspace, size_t workspace_bytes, int32_
  * enabled, every call records HIP events on its stream before and after each
  * of its kernels (up to `capacity` calls, then recording stops).
  * speedy_timer_read synchronizes on the events and returns, per kernel kind
- * (0 = level-set/latitude-table kernel, 1 = forward kernel, 2 = forward+VJP
- * kernel), the summed milliseconds and launch count; it then clears them. */
+ * (0 = level-set kernel, 1 = forward kernel, 2 = forward+VJP kernel, 3 =
+ * parameter-gradient reduction), the summed milliseconds and launch count; it
+ * then clears them. */
 int speedy_timer_enable(int32_t capacity);
 int speedy_timer_disable(void);
 int speedy_timer_read(float *ms_per_kind, int32_t *launches_per_kind, int32_t nkinds);

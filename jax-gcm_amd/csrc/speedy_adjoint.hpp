@@ -25,6 +25,10 @@ namespace speedy {
 
 enum { VJP_RHBL = 0, VJP_TRCNV = 1, VJP_RHLSC = 2, VJP_CO2 = 3, VJP_DRAG = 4, VJP_NPARAM = 5 };
 
+// Floats per column of the longwave tape store used by column_vjp.
+template <int KX>
+constexpr int vjp_tape_floats() { return 4 * (KX + 1); }
+
 // Cotangents of one column's tendencies.  Only level KX-1 of the u and v
 // tendencies is nonzero in the forward (surface stress), so only those matter.
 template <typename R, int KX>
@@ -84,24 +88,87 @@ SPD_HD R radset4_adj(R temp, R omeps, const R a_fb[4]) {
   return a_jt * dmin_x(jlo, R(320.0)) * dmax_x(temp, R(200.0));
 }
 
-// ---------------------------------------------------------------- longwave up
+// Adjoint of level k's longwave transmissivities tauL[k][:] = exp(-ab * ps * dhs)
+// (shortwave_radiation.py:177-200) and emissions st4a0[k], st4a1[k]
+// (longwave_radiation.py:48-68), applied at once so that no per-level adjoint
+// arrays stay live.  Gradients of the half-level temperatures go to a_Th.
 template <typename R, int KX>
-SPD_HD void lw_up_adj(const ColumnIn<R, KX>& c, const PhaseB<R, KX>& b, const SpeedyParams& p,
-                      const SpeedyVertical& v, const Uniforms<R>& u, const LwuTape<R, KX>& t, const R a_dfl[KX],
-                      R a_tau[KX][4], R a_st4a0[KX], R a_st4a1[KX], R& a_rlus2, R a_flux_h[2],
-                      ColumnAdj<R, KX>& g) {
+SPD_HD void lw_level_adj(int k, const R a_tau[4], R a_s0, R a_s1, const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a,
+                         const PhaseB<R, KX>& b, const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u,
+                         ColumnAdj<R, KX>& g, R a_Th[KX]) {
+  R ab[4];
+  lw_absorptivity<R, KX>(k, c, a, p, u, ab);
+#pragma unroll
+  for (int bnd = 0; bnd < 4; ++bnd) {
+    const R ax = a_tau[bnd] * b.tauL[k][bnd];
+    g.ps += ax * (-(ab[bnd] * v.dhs[k]));
+    const R a_ab = ax * (-(c.ps * v.dhs[k]));
+    if (bnd == 0) {
+      if (k >= 2 && k <= KX - 2) g.cloudc += a_ab * ((k > a.icltop) ? p.ablcl1 : p.ablcl2);
+    } else if (bnd == 1) {
+      g.prm[VJP_CO2] += a_ab * u.ablco2_d;
+    } else if (k >= 1) {
+      const R wv = (bnd == 2) ? p.ablwv1 : p.ablwv2;
+      if (k >= 2 && k <= KX - 2) {
+        const R x = wv * c.Q[k], ac2 = a.cloudc * p.ablcl2;
+        g.Q[k] += a_ab * dmax_x(x, ac2) * wv;
+        g.cloudc += a_ab * dmax_x(ac2, x) * p.ablcl2;
+      } else {
+        g.Q[k] += a_ab * wv;
+      }
+    }
+  }
+  // emissions; Th[j] = T[j] + wvi1[j] * (T[j+1] - T[j]) as in lw_down
+  auto Th = [&](int j) { return c.T[j] + u.wvi1[j] * (c.T[j + 1] - c.T[j]); };
+  if (k >= 2) {
+    const R T = c.T[k];
+    const R st3a = R(SBC) * pow3(T);
+    R grad;
+    if (k < KX - 1) grad = R(0.5) * mmax(Th(k) - Th(k - 1), R(0.0));
+    else grad = mmax(c.T[KX - 1] - Th(KX - 2), R(0.0));
+    const R a_st3a = a_s0 * T + a_s1 * R(4.0) * grad;
+    g.T[k] += a_s0 * st3a + a_st3a * R(3.0 * SBC) * T * T;
+    const R a_grad = a_s1 * R(4.0) * st3a;
+    if (k < KX - 1) {
+      const R ax = a_grad * R(0.5) * dmax_x(Th(k) - Th(k - 1), R(0.0));
+      a_Th[k] += ax;
+      a_Th[k - 1] += -ax;
+    } else {
+      const R ax = a_grad * dmax_x(c.T[KX - 1] - Th(KX - 2), R(0.0));
+      g.T[KX - 1] += ax;
+      a_Th[KX - 2] += -ax;
+    }
+  } else if (k == 0) {
+    const R s0 = R(0.75) * c.T[0] + R(0.25) * Th(0);
+    const R as = a_s0 * R(4.0 * SBC) * pow3(s0);
+    g.T[0] += R(0.75) * as;
+    a_Th[0] += R(0.25) * as;
+  } else {
+    const R s1 = R(0.50) * c.T[1] + R(0.25) * (Th(0) + Th(1));
+    const R as = a_s0 * R(4.0 * SBC) * pow3(s1);
+    g.T[1] += R(0.50) * as;
+    a_Th[0] += R(0.25) * as;
+    a_Th[1] += R(0.25) * as;
+  }
+}
+
+// ---------------------------------------------------------------- longwave up
+template <typename R, int KX, class FuAt>
+SPD_HD void lw_up_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b,
+                      const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, FuAt Fu,
+                      const R a_dfl[KX], R& a_rlus2, R a_flux_h[2], ColumnAdj<R, KX>& g, R a_Th[KX]) {
   const R ps = c.ps;
   const R stratc1 = u.eps1 * ps;
   // corrections of the two top levels
   const R a_c1 = -a_dfl[0], a_c2 = -a_dfl[1];
-  a_st4a0[0] += a_c1 * v.dhs[0] * stratc1;
-  a_st4a0[1] += a_c2 * v.dhs[1] * stratc1;
   const R a_strat1 = a_c1 * v.dhs[0] * b.st4a0[0] + a_c2 * v.dhs[1] * b.st4a0[1];
   g.ps += a_c1 * c.stratz + a_strat1 * u.eps1;
   // top level: dfl[0] += (F0 - ft0) + (F1 - ft1), F = Fu[1]
   R aF[4];
   {
     R afb[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+    R atau[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+    R as0 = a_c1 * v.dhs[0] * stratc1, as1 = R(0.0);
     R fb[4];
     radset4(c.T[0], u.omeps, fb);
 #pragma unroll
@@ -109,30 +176,33 @@ SPD_HD void lw_up_adj(const ColumnIn<R, KX>& c, const PhaseB<R, KX>& b, const Sp
       const R a_ft = -a_dfl[0];
       const R tau = b.tauL[0][bnd], em = R(1.0) - tau;
       aF[bnd] = a_dfl[0] + a_ft * tau;
-      a_tau[0][bnd] += a_ft * (t.Fu[1][bnd] - fb[bnd] * (b.st4a0[0] - R(2.0) * em * b.st4a1[0]));
+      atau[bnd] = a_ft * (Fu(1, bnd) - fb[bnd] * (b.st4a0[0] - R(2.0) * em * b.st4a1[0]));
       afb[bnd] = a_ft * em * (b.st4a0[0] - em * b.st4a1[0]);
-      a_st4a0[0] += a_ft * em * fb[bnd];
-      a_st4a1[0] += -a_ft * em * em * fb[bnd];
+      as0 += a_ft * em * fb[bnd];
+      as1 += -a_ft * em * em * fb[bnd];
     }
     aF[2] = R(0.0); aF[3] = R(0.0);
     g.T[0] += radset4_adj(c.T[0], u.omeps, afb);
+    lw_level_adj<R, KX>(0, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
   }
   // levels 1 .. KX-1 (the forward ran from KX-1 down to 1)
 #pragma unroll
   for (int k = 1; k < KX; ++k) {
-    R fb[4], afb[4];
+    R fb[4], afb[4], atau[4];
+    R as0 = (k == 1) ? a_c2 * v.dhs[1] * stratc1 : R(0.0), as1 = R(0.0);
     radset4(c.T[k], u.omeps, fb);
 #pragma unroll
     for (int bnd = 0; bnd < 4; ++bnd) {
       const R tau = b.tauL[k][bnd], em = R(1.0) - tau;
       const R a = aF[bnd] - a_dfl[k];            // dfl[k] += sum Fu[k+1] - sum Fu[k]
-      a_tau[k][bnd] += a * (t.Fu[k + 1][bnd] - fb[bnd] * (b.st4a0[k] - R(2.0) * em * b.st4a1[k]));
+      atau[bnd] = a * (Fu(k + 1, bnd) - fb[bnd] * (b.st4a0[k] - R(2.0) * em * b.st4a1[k]));
       afb[bnd] = a * em * (b.st4a0[k] - em * b.st4a1[k]);
-      a_st4a0[k] += a * em * fb[bnd];
-      a_st4a1[k] += -a * em * em * fb[bnd];
+      as0 += a * em * fb[bnd];
+      as1 += -a * em * em * fb[bnd];
       aF[bnd] = a_dfl[k] + a * tau;
     }
     g.T[k] += radset4_adj(c.T[k], u.omeps, afb);
+    lw_level_adj<R, KX>(k, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
   }
   // surface source: Fu[KX] = fbt * rlus2 + refsfc * flux_h; dfl[KX-1] += epslw * rlus2
   R fbt[4];
@@ -418,118 +488,60 @@ SPD_HD void shortwave_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, cons
 }
 
 // ---------------------------------------------------------------- longwave down
-template <typename R, int KX>
-SPD_HD void lw_down_adj(const ColumnIn<R, KX>& c, const PhaseB<R, KX>& b, const SpeedyParams& p,
-                        const Uniforms<R>& u, const LwdTape<R, KX>& t, const R a_dfl[KX], R a_rlds,
-                        R a_tau[KX][4], R a_st4a0[KX], R a_st4a1[KX], ColumnAdj<R, KX>& g) {
+template <typename R, int KX, class FdAt>
+SPD_HD void lw_down_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b,
+                        const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, FdAt Fd,
+                        const R a_dfl[KX], R a_rlds, ColumnAdj<R, KX>& g, R a_Th[KX]) {
   // rlds = sum_b emisfc * Fd[KX-1][b] + corlw; dfl[KX-1] -= corlw; corlw = corlw_u * st4a0[KX-1]
-  a_st4a0[KX - 1] += (a_rlds - a_dfl[KX - 1]) * u.corlw;
+  const R a_corlw = (a_rlds - a_dfl[KX - 1]) * u.corlw;
   R aF[4];
 #pragma unroll
   for (int bnd = 0; bnd < 4; ++bnd) aF[bnd] = a_rlds * p.emisfc;
 #pragma unroll
   for (int k = KX - 1; k >= 1; --k) {
-    R fb[4], afb[4];
+    R fb[4], afb[4], atau[4];
+    R as0 = (k == KX - 1) ? a_corlw : R(0.0), as1 = R(0.0);
     radset4(c.T[k], u.omeps, fb);
 #pragma unroll
     for (int bnd = 0; bnd < 4; ++bnd) {
       const R tau = b.tauL[k][bnd], em = R(1.0) - tau;
       const R a = aF[bnd] - a_dfl[k];           // dfl[k] = sum Fd[k-1] - sum Fd[k]
-      a_tau[k][bnd] += a * (t.Fd[k - 1][bnd] - fb[bnd] * (b.st4a0[k] + R(2.0) * em * b.st4a1[k]));
+      atau[bnd] = a * (Fd(k - 1, bnd) - fb[bnd] * (b.st4a0[k] + R(2.0) * em * b.st4a1[k]));
       afb[bnd] = a * em * (b.st4a0[k] + em * b.st4a1[k]);
-      a_st4a0[k] += a * em * fb[bnd];
-      a_st4a1[k] += a * em * em * fb[bnd];
+      as0 += a * em * fb[bnd];
+      as1 += a * em * em * fb[bnd];
       aF[bnd] = a_dfl[k] + a * tau;
     }
     g.T[k] += radset4_adj(c.T[k], u.omeps, afb);
+    lw_level_adj<R, KX>(k, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
   }
   {
     R fb[4];
     radset4(c.T[0], u.omeps, fb);
     R afb[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+    R atau[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+    R as0 = R(0.0), as1 = R(0.0);
 #pragma unroll
     for (int bnd = 0; bnd < 2; ++bnd) {
       const R tau = b.tauL[0][bnd], em = R(1.0) - tau;
       const R a = aF[bnd] - a_dfl[0];
-      a_tau[0][bnd] += -a * fb[bnd] * (b.st4a0[0] + R(2.0) * em * b.st4a1[0]);
+      atau[bnd] = -a * fb[bnd] * (b.st4a0[0] + R(2.0) * em * b.st4a1[0]);
       afb[bnd] = a * em * (b.st4a0[0] + em * b.st4a1[0]);
-      a_st4a0[0] += a * em * fb[bnd];
-      a_st4a1[0] += a * em * em * fb[bnd];
+      as0 += a * em * fb[bnd];
+      as1 += a * em * em * fb[bnd];
     }
     g.T[0] += radset4_adj(c.T[0], u.omeps, afb);
+    lw_level_adj<R, KX>(0, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
   }
 }
 
-// Blackbody emission st4a (longwave_radiation.py:48-68) back to temperature.
+// Half-level temperatures Th[k] = T[k] + wvi1[k] * (T[k+1] - T[k]) back to T.
 template <typename R, int KX>
-SPD_HD void st4a_adj(const ColumnIn<R, KX>& c, const Uniforms<R>& u, const LwdTape<R, KX>& t, const R a_st4a0[KX],
-                     const R a_st4a1[KX], ColumnAdj<R, KX>& g) {
-  R a_Th[KX];
-#pragma unroll
-  for (int k = 0; k < KX; ++k) a_Th[k] = R(0.0);
-#pragma unroll
-  for (int k = 2; k < KX; ++k) {
-    const R T = c.T[k];
-    const R st3a = R(SBC) * pow3(T);
-    const R a_st3a = a_st4a0[k] * T + a_st4a1[k] * R(4.0) * t.grad[k];
-    g.T[k] += a_st4a0[k] * st3a + a_st3a * R(3.0 * SBC) * T * T;
-    const R a_grad = a_st4a1[k] * R(4.0) * st3a;
-    if (k < KX - 1) {
-      const R x = t.Th[k] - t.Th[k - 1];
-      const R ax = a_grad * R(0.5) * dmax_x(x, R(0.0));
-      a_Th[k] += ax;
-      a_Th[k - 1] += -ax;
-    } else {
-      const R x = c.T[KX - 1] - t.Th[KX - 2];
-      const R ax = a_grad * dmax_x(x, R(0.0));
-      g.T[KX - 1] += ax;
-      a_Th[KX - 2] += -ax;
-    }
-  }
-  const R a_s0 = a_st4a0[0] * R(4.0 * SBC) * pow3(t.s0);
-  const R a_s1 = a_st4a0[1] * R(4.0 * SBC) * pow3(t.s1);
-  g.T[0] += R(0.75) * a_s0;
-  a_Th[0] += R(0.25) * a_s0;
-  g.T[1] += R(0.50) * a_s1;
-  a_Th[0] += R(0.25) * a_s1;
-  a_Th[1] += R(0.25) * a_s1;
+SPD_HD void half_level_adj(const Uniforms<R>& u, const R a_Th[KX], ColumnAdj<R, KX>& g) {
 #pragma unroll
   for (int k = 0; k < KX - 1; ++k) {
     g.T[k] += a_Th[k] * (R(1.0) - u.wvi1[k]);
     g.T[k + 1] += a_Th[k] * u.wvi1[k];
-  }
-}
-
-// Longwave transmissivities tauL = exp(-ab * ps * dhs) back to q, ps, cloud cover
-// and the CO2 scale.
-template <typename R, int KX>
-SPD_HD void lw_tau_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b,
-                       const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, R a_tau[KX][4],
-                       ColumnAdj<R, KX>& g) {
-#pragma unroll
-  for (int k = 0; k < KX; ++k) {
-    R ab[4];
-    lw_absorptivity<R, KX>(k, c, a, p, u, ab);
-#pragma unroll
-    for (int bnd = 0; bnd < 4; ++bnd) {
-      const R ax = a_tau[k][bnd] * b.tauL[k][bnd];
-      g.ps += ax * (-(ab[bnd] * v.dhs[k]));
-      const R a_ab = ax * (-(c.ps * v.dhs[k]));
-      if (bnd == 0) {
-        if (k >= 2 && k <= KX - 2) g.cloudc += a_ab * ((k > a.icltop) ? p.ablcl1 : p.ablcl2);
-      } else if (bnd == 1) {
-        g.prm[VJP_CO2] += a_ab * u.ablco2_d;
-      } else if (k >= 1) {
-        const R wv = (bnd == 2) ? p.ablwv1 : p.ablwv2;
-        if (k >= 2 && k <= KX - 2) {
-          const R x = wv * c.Q[k], ac2 = a.cloudc * p.ablcl2;
-          g.Q[k] += a_ab * dmax_x(x, ac2) * wv;
-          g.cloudc += a_ab * dmax_x(ac2, x) * p.ablcl2;
-        } else {
-          g.Q[k] += a_ab * wv;
-        }
-      }
-    }
   }
 }
 
@@ -636,9 +648,15 @@ SPD_HD void condensation_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, c
                              const ColumnCt<R, KX>& ct, ColumnAdj<R, KX>& g) {
   const R ps = c.ps;
   const R psa2 = ps * ps;
+  R qlsc[KX];
+#pragma unroll
+  for (int k = 1; k < KX; ++k) {
+    const R dqa = u.rhref[k] * a.qsat[k] - c.Q[k];
+    qlsc[k] = dqa < R(0.0) ? dqa * u.rtlsc : R(0.0);
+  }
   R sum = R(0.0);
 #pragma unroll
-  for (int k = 1; k < KX; ++k) sum = sum + u.pfact[k] * a.qlsc[k];
+  for (int k = 1; k < KX; ++k) sum = (k == 1) ? u.pfact[k] * qlsc[k] : sum + u.pfact[k] * qlsc[k];
   g.ps += g.precls * (R(0.0) - sum);
   const R a_sum = -g.precls * ps;
   R a_psa2 = R(0.0);
@@ -646,7 +664,7 @@ SPD_HD void condensation_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, c
   for (int k = 1; k < KX; ++k) {
     const R dqa = u.rhref[k] * a.qsat[k] - c.Q[k];
     if (dqa < R(0.0)) {
-      const R dq = a.qlsc[k];
+      const R dq = qlsc[k];
       const R cap = u.dqmax[k] * psa2;
       const R a_m = ct.dt[k] * R(TFACT);
       const R a_dq = ct.dq[k] + a_sum * u.pfact[k] - a_m * dmin_x(-dq, cap);
@@ -851,22 +869,42 @@ SPD_HD void humidity_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const
   g.ps += -g.rps * rps * rps;
 }
 
+// Per-column storage for the longwave flux tapes between a term's recompute and
+// its reversal: a plain array on the host; LDS on the GPU (speedy_kernels.hip).
+template <typename R, int N>
+struct ArrayStore {
+  R v[N];
+  SPD_HD void put(int i, R x) { v[i] = x; }
+  SPD_HD R get(int i) const { return v[i]; }
+};
+
 // The whole column: given the forward's phase outputs (a: phase A, b: phase B
 // with the longwave-down divergence still in b.dfl and the hand-offs flux_h,
 // rlus2, tsfc) and the cotangents, accumulate the gradients into g.
-template <typename R, int KX>
-SPD_HD void column_vjp(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b, uint32_t S,
+template <typename R, int KX, class Store>
+SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b, uint32_t S,
                        const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, uint32_t flags,
-                       const ColumnCt<R, KX>& ct, ColumnAdj<R, KX>& g) {
+                       const ColumnCt<R, KX>& ct, ColumnAdj<R, KX>& g, Store& tape) {
   adj_zero<R, KX>(g);
+  // The terms below recompute the forward from the column.  Hide the column's
+  // values from the optimizer so it recomputes instead of keeping every forward
+  // intermediate live across the adjoint (they are equal by construction).
+  ColumnIn<R, KX> c = c_in;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    asm volatile("" : "+v"(c.T[k]));
+    asm volatile("" : "+v"(c.Q[k]));
+    asm volatile("" : "+v"(c.PHI[k]));
+  }
+  asm volatile("" : "+v"(c.ps));
+#endif
   const R rps = R(1.0) / c.ps;
-  R a_dfl[KX], a_tau[KX][4], a_st4a0[KX], a_st4a1[KX];
+  R a_dfl[KX], a_Th[KX];
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
     a_dfl[k] = ct.dt[k] * v.grdscp[k] * rps;
-    a_st4a0[k] = R(0.0); a_st4a1[k] = R(0.0);
-#pragma unroll
-    for (int bnd = 0; bnd < 4; ++bnd) a_tau[k][bnd] = R(0.0);
+    a_Th[k] = R(0.0);
   }
   R a_rlus2 = R(0.0), a_flux_h[2] = {R(0.0), R(0.0)}, a_rsds = R(0.0), a_rlds = R(0.0);
   {
@@ -876,7 +914,12 @@ SPD_HD void column_vjp(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const P
     lw_up<R, KX, false>(c, p, v, u, bb, b.rlus2, b.tsfc, fh, lt);
 #pragma unroll
     for (int k = 0; k < KX; ++k) g.rps += ct.dt[k] * lt.dfl[k] * v.grdscp[k];
-    lw_up_adj<R, KX>(c, b, p, v, u, lt, a_dfl, a_tau, a_st4a0, a_st4a1, a_rlus2, a_flux_h, g);
+#pragma unroll
+    for (int k = 1; k <= KX; ++k)
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) tape.put(k * 4 + bnd, lt.Fu[k][bnd]);
+    lw_up_adj<R, KX>(c, a, b, p, v, u, [&](int k, int bnd) { return tape.get(k * 4 + bnd); }, a_dfl, a_rlus2,
+                     a_flux_h, g, a_Th);
   }
   {
     PhaseB<R, KX> bb = b;
@@ -896,10 +939,14 @@ SPD_HD void column_vjp(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const P
     PhaseB<R, KX> bb = b;
     LwdTape<R, KX> lt;
     lw_down<R, KX>(c, p, u, bb, lt);
-    lw_down_adj<R, KX>(c, b, p, u, lt, a_dfl, a_rlds, a_tau, a_st4a0, a_st4a1, g);
-    st4a_adj<R, KX>(c, u, lt, a_st4a0, a_st4a1, g);
+#pragma unroll
+    for (int k = 0; k < KX - 1; ++k)
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) tape.put(k * 4 + bnd, lt.Fd[k][bnd]);
+    lw_down_adj<R, KX>(c, a, b, p, v, u, [&](int k, int bnd) { return tape.get(k * 4 + bnd); }, a_dfl, a_rlds, g,
+                       a_Th);
   }
-  lw_tau_adj<R, KX>(c, a, b, p, v, u, a_tau, g);
+  half_level_adj<R, KX>(u, a_Th, g);
   vdiff_adj<R, KX>(c, a, p, u, ct, g);
   {
     PhaseA<R, KX> aa = a;

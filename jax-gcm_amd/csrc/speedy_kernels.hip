@@ -33,7 +33,7 @@
 #include <string>
 #include <vector>
 
-#include "speedy_column.hpp"
+#include "speedy_adjoint.hpp"
 
 namespace speedy {
 
@@ -56,6 +56,9 @@ struct KArgs {
   int32_t level_set;
   uint32_t flags, term_mask;
   uint32_t *gen, *err, *last_s, *partial;
+  // fused forward + VJP only
+  SpeedyAdjoint adj;
+  float* prm_part;                   // parameter-gradient partials (per block and member slot, or per column)
 };
 
 // ---------------------------------------------------------------- diagnostics
@@ -263,9 +266,80 @@ __global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
   for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[blockIdx.x * a.jmax + i] = s_or[i];
 }
 
-// Forward.  FUSED: single launch with the split grid barrier (all blocks
-// co-resident); otherwise S comes from levelset_kernel of the same call.
-template <int KX, bool DIAG, bool FUSED>
+// ---------------------------------------------------------------- VJP helpers
+// One column's slice of an LDS array laid out [slot][thread] (conflict-free).
+struct LdsStore {
+  float* base;
+  __device__ __forceinline__ void put(int i, float x) { base[i * kBlock] = x; }
+  __device__ __forceinline__ float get(int i) const { return base[i * kBlock]; }
+};
+
+template <int KX>
+__device__ __forceinline__ void load_cotangents(const KArgs& a, uint32_t m, uint32_t c, ColumnCt<float, KX>& ct) {
+  const size_t ncol = a.ncol;
+  const size_t base3 = (size_t)m * KX * ncol + c;
+  ct.du1 = a.adj.ct_du[base3 + (KX - 1) * ncol];
+  ct.dv1 = a.adj.ct_dv[base3 + (KX - 1) * ncol];
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    ct.dt[k] = a.adj.ct_dt[base3 + k * ncol];
+    ct.dq[k] = a.adj.ct_dq[base3 + k * ncol];
+  }
+}
+
+template <int KX>
+__device__ __forceinline__ void store_grads(const KArgs& a, uint32_t m, uint32_t c, const ColumnAdj<float, KX>& g) {
+  const size_t ncol = a.ncol;
+  const size_t base3 = (size_t)m * KX * ncol + c;
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    a.adj.g_u[base3 + k * ncol] = k == KX - 1 ? g.u1 : 0.0f;
+    a.adj.g_v[base3 + k * ncol] = k == KX - 1 ? g.v1 : 0.0f;
+    a.adj.g_t[base3 + k * ncol] = g.T[k];
+    a.adj.g_q[base3 + k * ncol] = g.Q[k];
+  }
+  a.adj.g_ps[(size_t)m * ncol + c] = g.ps;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Parameter gradients of this block: a fixed-order tree sum per member slot
+// (deterministic), written to prm_part[block][slot][5].  Members narrower than
+// a block write per-column values instead (prm_part[column][5]).
+__device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t gid, bool active, uint32_t slot,
+                                                     const float prm[VJP_NPARAM]) {
+  if (a.ncol < (uint32_t)kBlock) {
+    if (active)
+#pragma unroll
+      for (int i = 0; i < VJP_NPARAM; ++i) a.prm_part[(size_t)gid * VJP_NPARAM + i] = prm[i];
+    return;
+  }
+  __shared__ float s_red[kBlock / 64][2 * VJP_NPARAM];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < VJP_NPARAM; ++i) {
+      const float v = wave_sum((active && slot == (uint32_t)j) ? prm[i] : 0.0f);
+      if (lane == 0) s_red[wave][j * VJP_NPARAM + i] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < 2 * VJP_NPARAM) {
+    float acc = s_red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) acc = acc + s_red[w][threadIdx.x];
+    a.prm_part[(size_t)blockIdx.x * 2 * VJP_NPARAM + threadIdx.x] = acc;
+  }
+}
+
+// Forward (and, with VJP, the fused forward + VJP).  FUSED: single launch with
+// the split grid barrier (all blocks co-resident); otherwise S comes from
+// levelset_kernel of the same call.
+template <int KX, bool DIAG, bool FUSED, bool VJP>
 __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
   __shared__ uint32_t s_or[2], s_s[2], s_tag;
   const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
@@ -330,6 +404,52 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
   load_zonal<KX>(a, c, col);
   phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
   if (active) store_outputs<KX, DIAG>(a, m, c, col, pa, pb);
+  if (VJP) {
+    ColumnCt<float, KX> ct;
+    load_cotangents<KX>(a, m, c, ct);
+    ColumnAdj<float, KX> g;
+    // longwave flux tapes in LDS, [slot][thread]: the kernel runs one block per CU
+    __shared__ float s_tape[vjp_tape_floats<KX>() * kBlock];
+    LdsStore tape{s_tape + threadIdx.x};
+    column_vjp<float, KX>(col, pa, pb, S, a.p, a.v, a.u, a.flags, ct, g, tape);
+    if (active) store_grads<KX>(a, m, c, g);
+    block_param_partials(a, gid, active, m - m_first, g.prm);
+  }
+}
+
+// Per-member parameter gradients from the partials, one block per member, in a
+// fixed order (deterministic).
+__global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) {
+  const uint32_t m = blockIdx.x;
+  float acc[VJP_NPARAM];
+#pragma unroll
+  for (int i = 0; i < VJP_NPARAM; ++i) acc[i] = 0.0f;
+  if (a.ncol >= (uint32_t)kBlock) {
+    const uint32_t b0 = (m * a.ncol) / kBlock, b1 = (m * a.ncol + a.ncol - 1u) / kBlock;
+    for (uint32_t b = b0 + threadIdx.x; b <= b1; b += kBlock) {
+      const uint32_t j = m - divq(b * kBlock, a.dcol);
+#pragma unroll
+      for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += a.prm_part[((size_t)b * 2 + j) * VJP_NPARAM + i];
+    }
+  } else {
+    for (uint32_t cc = threadIdx.x; cc < a.ncol; cc += kBlock)
+#pragma unroll
+      for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += a.prm_part[((size_t)m * a.ncol + cc) * VJP_NPARAM + i];
+  }
+  __shared__ float s_red[kBlock / 64][VJP_NPARAM];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int i = 0; i < VJP_NPARAM; ++i) {
+    const float v = wave_sum(acc[i]);
+    if (lane == 0) s_red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < VJP_NPARAM) {
+    float t = s_red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) t = t + s_red[w][threadIdx.x];
+    a.adj.g_params[(size_t)m * VJP_NPARAM + threadIdx.x] = t;
+  }
 }
 
 __global__ void fill_zero_kernel(uint32_t* p, int64_t n) {
@@ -391,7 +511,7 @@ Grid make_grid(int64_t nlon, int64_t nlat, int64_t nmember) {
 }
 
 struct WsLayout {
-  size_t last_s, partial, total;
+  size_t last_s, partial, prm, total;
 };
 
 WsLayout ws_layout(const Grid& g, int32_t nmember) {
@@ -399,7 +519,9 @@ WsLayout ws_layout(const Grid& g, int32_t nmember) {
   L.last_s = 256;
   L.partial = L.last_s + align256(sizeof(uint32_t) * (size_t)nmember);
   const size_t npart = std::max<size_t>(2 * (size_t)g.nblk, (size_t)g.ls_blocks * g.jmax);
-  L.total = L.partial + align256(sizeof(uint32_t) * npart);
+  L.prm = L.partial + align256(sizeof(uint32_t) * npart);
+  const size_t nprm = g.ncol >= (uint32_t)kBlock ? (size_t)g.nblk * 2 * VJP_NPARAM : (size_t)g.ntot * VJP_NPARAM;
+  L.total = L.prm + align256(sizeof(float) * nprm);
   return L;
 }
 
@@ -459,6 +581,7 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   a.err = a.gen + 1;
   a.last_s = reinterpret_cast<uint32_t*>(w + L.last_s);
   a.partial = reinterpret_cast<uint32_t*>(w + L.partial);
+  a.prm_part = reinterpret_cast<float*>(w + L.prm);
   return SPEEDY_OK;
 }
 
@@ -472,24 +595,25 @@ int64_t coresident_blocks(K kernel) {
   return (int64_t)per_cu * ncu;
 }
 
-template <int KX, bool DIAG>
+template <int KX, bool DIAG, bool VJP>
 int launch_fwd(const KArgs& a, hipStream_t s) {
   static int64_t cap = -1;   // one GPU per process in this library's deployments
-  if (cap < 0) cap = coresident_blocks(fwd_kernel<KX, DIAG, true>);
+  if (cap < 0) cap = coresident_blocks(fwd_kernel<KX, DIAG, true, VJP>);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(s, &cs);
   const bool fused = a.level_set < 0 && (int64_t)a.nblk <= cap && a.ncol >= (uint32_t)kBlock;
+  const int kind = VJP ? 2 : 1;
   hipError_t e = hipSuccess;
   if (fused) {
-    const int slot = timer_begin(s, 1);
+    const int slot = timer_begin(s, kind);
     if (cs == hipStreamCaptureStatusNone) {
       void* args[] = {const_cast<KArgs*>(&a)};
-      e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&fwd_kernel<KX, DIAG, true>), dim3(a.nblk),
+      e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&fwd_kernel<KX, DIAG, true, VJP>), dim3(a.nblk),
                                      dim3(kBlock), args, 0, s);
     } else {
       // Graph capture: a plain launch of a grid that fits on the device; the
       // level-set poll is bounded, so the kernel terminates in every case.
-      hipLaunchKernelGGL((fwd_kernel<KX, DIAG, true>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL((fwd_kernel<KX, DIAG, true, VJP>), dim3(a.nblk), dim3(kBlock), 0, s, a);
       e = hipGetLastError();
     }
     timer_end(s, slot);
@@ -500,8 +624,14 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
       hipLaunchKernelGGL(levelset_kernel<KX>, dim3(a.ls_blocks), dim3(kBlock), a.jmax * sizeof(uint32_t), s, a);
       timer_end(s, slot);
     }
-    slot = timer_begin(s, 1);
-    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+    slot = timer_begin(s, kind);
+    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false, VJP>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+    timer_end(s, slot);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && VJP) {
+    const int slot = timer_begin(s, 3);
+    hipLaunchKernelGGL(param_reduce_kernel, dim3(a.nmember), dim3(kBlock), 0, s, a);
     timer_end(s, slot);
     e = hipGetLastError();
   }
@@ -511,7 +641,7 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
 
 template <int KX>
 int launch_fwd_kx(const KArgs& a, hipStream_t s) {
-  return a.f.diag ? launch_fwd<KX, true>(a, s) : launch_fwd<KX, false>(a, s);
+  return a.f.diag ? launch_fwd<KX, true, false>(a, s) : launch_fwd<KX, false, false>(a, s);
 }
 
 }  // namespace speedy
@@ -580,6 +710,29 @@ int speedy_fwd(const SpeedyFields* fields, const SpeedyVertical* vert, const Spe
     case 7: return launch_fwd_kx<7>(a, s);
     case 8: return launch_fwd_kx<8>(a, s);
     default: return launch_fwd_kx<16>(a, s);
+  }
+}
+
+int speedy_fwd_vjp(const SpeedyFields* fields, const SpeedyAdjoint* adj, const SpeedyVertical* vert,
+                   const SpeedyParams* params, const SpeedyDate* date, int32_t nlon, int32_t nlat, int32_t nmember,
+                   uint32_t flags, int32_t level_set, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!adj) return fail(SPEEDY_ERR_ARG, "speedy: NULL adjoint struct");
+  const void* req[] = {adj->ct_du, adj->ct_dv, adj->ct_dt, adj->ct_dq, adj->g_u, adj->g_v,
+                       adj->g_t, adj->g_q, adj->g_ps, adj->g_params};
+  for (const void* p : req)
+    if (!p) return fail(SPEEDY_ERR_ARG, "speedy: a required adjoint pointer is NULL");
+  if (fields && fields->diag) return fail(SPEEDY_ERR_ARG, "speedy: fwd_vjp writes no diagnostics (diag must be NULL)");
+  KArgs a;
+  if (int rc = prepare(fields, vert, params, date, nlon, nlat, nmember, flags, SPEEDY_TERMS_ALL, level_set,
+                       workspace, workspace_bytes, a))
+    return rc;
+  a.adj = *adj;
+  g_last_error.clear();
+  hipStream_t s = (hipStream_t)stream;
+  switch (vert->kx) {
+    case 7: return launch_fwd<7, false, true>(a, s);
+    case 8: return launch_fwd<8, false, true>(a, s);
+    default: return launch_fwd<16, false, true>(a, s);
   }
 }
 

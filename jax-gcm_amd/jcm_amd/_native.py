@@ -139,7 +139,7 @@ def diag_names(kx: int) -> list[str]:
     return out
 
 
-KERNEL_KINDS = ("levelset", "fwd", "fwd_vjp")
+KERNEL_KINDS = ("levelset", "fwd", "fwd_vjp", "param_reduce")
 
 
 def timer_read():

@@ -127,7 +127,8 @@ static void run_vjp(const R* const* in, const R* const* ct, R* const* out, R* co
     cc.dv1 = ct[1][(KX - 1) * ncol + c];
     for (int k = 0; k < KX; ++k) { cc.dt[k] = ct[2][k * ncol + c]; cc.dq[k] = ct[3][k * ncol + c]; }
     ColumnAdj<R, KX> ga;
-    column_vjp<R, KX>(col, a, b, S, *p, *v, un, flags, cc, ga);
+    ArrayStore<R, vjp_tape_floats<KX>()> tape;
+    column_vjp<R, KX>(col, a, b, S, *p, *v, un, flags, cc, ga, tape);
     for (int k = 0; k < KX; ++k) {
       g[0][k * ncol + c] = k == KX - 1 ? ga.u1 : R(0);
       g[1][k * ncol + c] = k == KX - 1 ? ga.v1 : R(0);

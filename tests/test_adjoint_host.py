@@ -163,3 +163,45 @@ def test_float32_adjoint_close_to_float64(name):
     for i in range(5):
         a, b = float(g32["prm"][..., i].astype(np.float64).sum()), float(g64["prm"][..., i].sum())
         assert abs(a - b) <= 1e-4 * max(abs(b), 1e-30) or b == 0, i
+
+
+def test_adjoint_configuration_t42_parameter_gradients_h1e3():
+    """The adjoint configuration: T42 (128 x 64), 8 levels, land recipe, the five
+    parameters drawn from their stated ranges, cotangents N(0,1).  Each parameter
+    gradient against a central difference of the float64 oracle with h = 1e-3 |p|,
+    as the configuration specifies, column by column.  A step of 1e-3 crosses
+    thresholds in some columns (the convective trigger on rhbl flips the convective
+    top: a jump, not a derivative), so a column counts only where the h = 1e-3
+    difference agrees with an h = 2.5e-4 one to 1e-3; those columns (>= 98 %)
+    must match the adjoint's per-column contributions, and their sums to 1e-4."""
+    case = synthetic.make_case(128, 64, 8, seed=2, land=True)
+    params = synthetic.draw_vjp_parameters(2)
+    kx, ix, il = case[2].nodal_shape
+    rng = np.random.default_rng(2)
+    cot = [rng.standard_normal((kx, ix, il)) for _ in range(4)]
+    _, _, S = H.run_host_column(case, params, dtype=np.float64)
+    _, g = H.run_host_vjp(case, params, cot, level_set=S)
+    for i, (grp, nm) in enumerate(VJP_PARAMS):
+        v0 = float(np.float32(getattr(getattr(params, grp), nm)))
+
+        def fd(hrel):
+            # parameters enter as float32: difference by the values actually used
+            vp, vm = float(np.float32(v0 * (1 + hrel))), float(np.float32(v0 * (1 - hrel)))
+            out = []
+            for vv in (vp, vm):
+                t, _, _ = H.run_oracle64_same_constants(case, params.replace(**{grp: {nm: vv}}), level_set=S)
+                out.append(_J(cot, [t.u_wind, t.v_wind, t.temperature, t.specific_humidity]))
+            return (out[0] - out[1]) / (vp - vm)
+
+        f3, f4 = fd(1e-3), fd(2.5e-4)
+        per_col = g["prm"][..., i]
+        scale = np.abs(per_col).max()
+        if scale == 0:
+            assert np.all(f3 == 0), nm
+            continue
+        ok = np.abs(f3 - f4) <= 1e-3 * np.abs(f4) + 1e-8 * scale
+        ours, ref = float(np.sum(per_col[ok])), float(np.sum(f3[ok]))
+        print(f"d/d {nm}: adjoint {ours:.8e}  oracle FD(h=1e-3) {ref:.8e}  columns with a jump {(~ok).sum()}")
+        assert (~ok).sum() <= 0.02 * ix * il
+        assert np.all(np.abs(per_col[ok] - f3[ok]) <= 2e-3 * np.abs(f3[ok]) + 1e-6 * scale), nm
+        assert abs(ours - ref) <= 1e-4 * max(abs(ref), 1e-12), nm

@@ -1,0 +1,177 @@
+"""GPU parity of the fused forward + VJP kernel (through the C ABI).
+
+References: the float64 host build of the same adjoint (pinned to central
+differences of the float64 oracle in test_adjoint_host) and, directly, central
+differences of the float64 oracle along a random direction.
+
+Bars (float32 kernel):
+  * gradient fields (T, q, u, v, ps): ||g_gpu - g_64||_inf / ||g_64||_inf <= 1e-4
+    (the float32 host adjoint is <= 5e-5 from float64);
+  * per-member parameter gradients: relative error <= 1e-4;
+  * the tendencies of the fused call: the forward's bar (1e-5 of the float32
+    oracle, or the float64 rule of helpers.parity).
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+from jcm_amd import Parameters, synthetic
+from jcm_amd.params import VJP_PARAMS
+from jcm_amd.physics_interface import PhysicsTendency
+from test_host_column import CASES, make
+
+pytestmark = pytest.mark.gpu
+GRAD_TOL = 1e-4
+FIELDS = ("u_wind", "v_wind", "temperature", "specific_humidity")
+GKEYS = dict(u_wind="u", v_wind="v", temperature="t", specific_humidity="q")
+
+
+@pytest.fixture(scope="module")
+def phys():
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    assert torch.cuda.is_available()
+    return SpeedyPhysics()
+
+
+def _np(x):
+    return x.detach().cpu().numpy()
+
+
+def _cot(shape, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal(shape).astype(np.float32) for _ in range(4)]
+
+
+def _gpu_vjp(phys, case, params, cot, level_set=-1):
+    import torch
+    phys.parameters = params
+    tend, sg, pg = phys.compute_tendencies_vjp(*case, PhysicsTendency(*cot), level_set=level_set)
+    torch.cuda.synchronize()
+    g = dict(u=_np(sg.u_wind), v=_np(sg.v_wind), t=_np(sg.temperature), q=_np(sg.specific_humidity),
+             ps=_np(sg.normalized_surface_pressure))
+    prm = np.stack([_np(pg[f"{grp}.{nm}"]) for grp, nm in VJP_PARAMS], axis=-1)
+    return tend, g, prm
+
+
+def _check_against_host(g, prm, g64, label):
+    for k in ("t", "q", "u", "v", "ps"):
+        e = H.rel_inf(g[k], g64[k])
+        print(f"{label} grad {k}: rel err vs float64 adjoint {e:.2e}", flush=True)
+        assert e <= GRAD_TOL, f"{label} {k}: {e:.3e}"
+    for i, (grp, nm) in enumerate(VJP_PARAMS):
+        ref = float(g64["prm"][..., i].sum())
+        e = abs(float(prm[i]) - ref) / max(abs(ref), 1e-30)
+        print(f"{label} d/d {nm}: gpu {float(prm[i]):.6e} float64 {ref:.6e} rel {e:.1e}", flush=True)
+        assert e <= GRAD_TOL or ref == 0.0 and prm[i] == 0.0, f"{label} {nm}: {e:.3e}"
+
+
+@pytest.mark.parametrize("name", ["T31_aqua", "T21_land", "T21_wet", "T21_wet_kx16", "T21_wet_kx7",
+                                  "T21_wet_swoff", "T21_wet_nofluxland"])
+def test_vjp_matches_float64_adjoint(phys, name):
+    params = synthetic.draw_vjp_parameters(2) if CASES[name]["wet"] else Parameters.default()
+    case = make(name)
+    kx, ix, il = case[2].nodal_shape
+    cot = _cot((kx, ix, il), 5)
+    tend, g, prm = _gpu_vjp(phys, case, params, cot)
+    S = phys.level_sets(ix, il, 1)[0]
+    _, _, S_ref = H.run_oracle(*case, params=params)
+    assert S == S_ref
+    _, g64 = H.run_host_vjp(case, params, cot, level_set=S)
+    _check_against_host(g, prm, g64, name)
+    # the fused call's tendencies meet the forward's bar
+    ref, _, _ = H.run_oracle(*case, params=params)
+    ref64, _, _ = H.run_oracle(*case, params=params, dtype=np.float64)
+    for f in FIELDS:
+        ok, e32, e64, eref = H.parity(_np(getattr(tend, f)), getattr(ref, f), getattr(ref64, f))
+        assert ok, f"{name} tendency {f}: {e32:.3e}"
+    assert phys.barrier_status(ix, il, 1) == 0
+
+
+def test_vjp_tendencies_equal_forward_kernel(phys):
+    """The forward part of the fused call is the forward kernel's code: same tendencies."""
+    import torch
+    case = make("T21_wet")
+    params = synthetic.draw_vjp_parameters(2)
+    kx, ix, il = case[2].nodal_shape
+    tend_v, _, _ = _gpu_vjp(phys, case, params, _cot((kx, ix, il), 1))
+    phys.parameters = params
+    tend_f, _ = phys.compute_tendencies(*case)
+    torch.cuda.synchronize()
+    for f in FIELDS:
+        e = H.rel_inf(_np(getattr(tend_v, f)), _np(getattr(tend_f, f)))
+        assert e <= 1e-6, (f, e)
+
+
+def test_vjp_t42_config_against_float64_oracle_direction(phys):
+    """T42 (128 x 64), 8 levels, land recipe, parameters drawn from the adjoint
+    configuration's ranges, cotangents N(0,1): <g, d> against the central
+    difference of the float64 oracle along a random direction d of the state and
+    of the five parameters (relative error <= 1e-4)."""
+    case = synthetic.make_case(128, 64, 8, seed=2, land=True)
+    params = synthetic.draw_vjp_parameters(2)
+    st, fo, ge, da = case
+    kx, ix, il = ge.nodal_shape
+    cot = _cot((kx, ix, il), 2)
+    _, g, prm = _gpu_vjp(phys, case, params, cot)
+    S = phys.level_sets(ix, il, 1)[0]
+    rng = np.random.default_rng(3)
+    d = dict(t=rng.standard_normal((kx, ix, il)), q=rng.standard_normal((kx, ix, il)) * 1e-2,
+             u=rng.standard_normal((kx, ix, il)), v=rng.standard_normal((kx, ix, il)),
+             ps=rng.standard_normal((ix, il)) * 1e-3)
+    dp = rng.standard_normal(5)
+    lin = sum(float(np.sum(g[k].astype(np.float64) * d[k])) for k in d) + float(np.dot(prm.astype(np.float64), dp))
+    h = 1e-6
+    J = lambda t: float(sum(np.sum(c.astype(np.float64) * x) for c, x in
+                            zip(cot, (t.u_wind, t.v_wind, t.temperature, t.specific_humidity))))
+
+    def shifted(sgn):
+        s = st.copy(temperature=st.temperature.astype(np.float64) + sgn * h * d["t"],
+                    specific_humidity=st.specific_humidity.astype(np.float64) + sgn * h * d["q"],
+                    u_wind=st.u_wind.astype(np.float64) + sgn * h * d["u"],
+                    v_wind=st.v_wind.astype(np.float64) + sgn * h * d["v"],
+                    normalized_surface_pressure=st.normalized_surface_pressure.astype(np.float64) + sgn * h * d["ps"])
+        sinp = H.oracle64_same_constants(s, fo, ge, da, params)
+        stt, foo, gg, dd, pp = sinp
+        for i, (grp, nm) in enumerate(VJP_PARAMS):
+            setattr(getattr(pp, grp), nm, getattr(getattr(pp, grp), nm) + sgn * h * dp[i])
+        from oracle import speedy_oracle as so
+        t, _, _ = so.compute_tendencies(stt, foo, gg, dd, pp, level_set=S, return_level_set=True)
+        return t
+
+    fd = (J(shifted(1)) - J(shifted(-1))) / (2 * h)
+    print(f"T42 directional derivative: gpu {lin:.8e} float64 oracle FD {fd:.8e}", flush=True)
+    assert abs(lin - fd) <= GRAD_TOL * abs(fd)
+
+
+def test_vjp_finite_at_unit_cotangents(phys):
+    case = make("T21_wet")
+    kx, ix, il = case[2].nodal_shape
+    ones = [np.ones((kx, ix, il), np.float32) for _ in range(4)]
+    _, g, prm = _gpu_vjp(phys, case, Parameters.default(), ones)
+    for k, v in g.items():
+        assert np.all(np.isfinite(v)), k
+    assert np.all(np.isfinite(prm))
+
+
+@pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3), (256, 128, 8), (8, 4, 6)])
+def test_vjp_members_stacked(phys, nlon, nlat, nmember):
+    """Stacked members (fused path; beyond one co-resident launch; narrower than a
+    workgroup): each member's gradients equal its own float64 adjoint, with its
+    own level set and its own parameter-gradient sum."""
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=9, land=True, nmember=nmember, member_t_sigma=1.0)
+    q = st.specific_humidity.copy()
+    q[1::2] *= np.float32(1.12)
+    st = st.copy(specific_humidity=q)
+    params = synthetic.draw_vjp_parameters(2)
+    cot = _cot((nmember, 8, nlon, nlat), 4)
+    _, g, prm = _gpu_vjp(phys, (st, fo, ge, da), params, cot)
+    S = phys.level_sets(nlon, nlat, nmember)
+    pick = lambda a, m: np.asarray(a)[m]
+    for m in range(nmember):
+        sm = st.copy(**{k: pick(getattr(st, k), m) for k in ("u_wind", "v_wind", "temperature", "specific_humidity",
+                                                             "geopotential", "normalized_surface_pressure")})
+        _, _, S_ref = H.run_oracle(sm, fo, ge, da, params=params)
+        assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
+        _, g64 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m])
+        _check_against_host({k: v[m] for k, v in g.items()}, prm[m], g64, f"member {m}")
