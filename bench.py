@@ -1,16 +1,28 @@
 """Benchmark: SPEEDY physics columns/s on one MI355X at T85 x 8 levels (FP32).
 
 Workload: T85 Gaussian grid, 256 x 128 = 32768 columns, 8 sigma levels, the
-land recipe of jcm_amd.synthetic (seed 1), inputs resident in HBM.  One step is
-one SpeedyPhysics.compute_tendencies call (the eleven SPEEDY terms; two kernel
-launches).  Steps are replayed from HIP graphs of --graph-calls calls so that
-host launch overhead does not hide the device time; K steps are bracketed by
-torch.cuda.synchronize() on both sides.
+land recipe of jcm_amd.synthetic (seed 1), inputs resident in HBM.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the fused
-forward kernel), timed live with HIP events the library records around its
-launches; `cpu_baseline` times the float32 NumPy oracle (the CPU restatement of
-the reference) on the same workload on one host core.
+  value      forward: one step is one SpeedyPhysics.compute_tendencies call (the
+             eleven SPEEDY terms, one fused kernel launch).  Steps are replayed
+             from HIP graphs of --graph-calls calls so that host launch overhead
+             does not hide device time; K steps are bracketed by
+             torch.cuda.synchronize() on both sides.
+  fwd_vjp    the same for compute_tendencies_vjp (fused forward + VJP w.r.t. T, q,
+             u, v, ps and five parameters, cotangents N(0,1); two launches: the
+             column kernel and the parameter-gradient sums).
+  per_call   both calls made from Python one by one without graphs (the plain
+             per-call path: host overhead included).
+  serving    a Poisson stream of requests of mixed resolution (T21, T31, T42,
+             T85) and kind (forward, forward + VJP) through the continuous-
+             batching loop (jcm_amd.serving) and through one call per request:
+             p50/p99 latency and columns/s.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the fused forward
+kernel, one launch per step): its average duration is the graph-replayed step
+time measured with HIP events on the replay stream; `cpu_baseline` times the
+float32 NumPy oracle (the CPU restatement of the reference) on the same
+workload on one host core.
 """
 from __future__ import annotations
 
@@ -25,10 +37,20 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "jax-gcm_amd")]
 
 METRIC = "SPEEDY physics columns/s fwd & fwd+VJP at T85 vs plain; p50/p99 req latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per column of the forward (kx = 8): reads T, q, phi at all
-# levels, u and v at the lowest level, ps, 6 forcing and 3 geometry fields
-# (3*kx + 12 floats); writes the 4 tendency fields (4*kx floats).
-FWD_BYTES_PER_COL = lambda kx: (3 * kx + 12 + 4 * kx) * 4
+
+
+def FWD_BYTES_PER_COL(kx):
+    """Forward: reads T, q, phi at all levels, u and v at the lowest level, ps,
+    6 forcing and 3 geometry fields (3*kx + 12 floats); writes the 4 tendency
+    fields (4*kx floats)."""
+    return (3 * kx + 12 + 4 * kx) * 4
+
+
+def VJP_BYTES_PER_COL(kx):
+    """Forward + VJP: the forward's reads plus the cotangents that matter (dT, dq
+    at all levels, du, dv at the lowest: 2*kx + 2); writes the 4 tendency fields
+    and the 4 state gradients (8*kx) and dL/dps (1)."""
+    return ((3 * kx + 12) + (2 * kx + 2) + (8 * kx + 1)) * 4
 
 
 def parse():
@@ -37,11 +59,15 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--graph-calls", type=int, default=50)
-    ap.add_argument("--timer-steps", type=int, default=500)
+    ap.add_argument("--per-call-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rates", type=str, default="2000,20000",
+                    help="Poisson arrival rates (requests/s) of the serving runs")
     ap.add_argument("--profile-only", action="store_true",
-                    help="replay --steps calls, run a copy of --copy-mib MiB, and exit (for rocprofv3)")
+                    help="replay --steps forward and fwd+VJP calls, run a copy of --copy-mib MiB, and exit")
     ap.add_argument("--copy-mib", type=int, default=256)
     return ap.parse_args()
 
@@ -75,6 +101,89 @@ def cpu_baseline(case, seconds):
             "sample": f"{n} full T85x8 forward calls ({n * ncol} columns) of the float32 NumPy oracle, {el:.1f} s"}
 
 
+class GraphStep:
+    """`calls` invocations of fn captured in one HIP graph on a side stream."""
+
+    def __init__(self, fn, calls, dev):
+        import torch
+        self.calls = calls
+        stream = torch.cuda.Stream(device=dev)
+        stream.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(stream):
+            fn()                                  # warm the side stream
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=stream):
+            for _ in range(calls):
+                fn()
+        torch.cuda.synchronize()
+        self.graph.replay()
+        torch.cuda.synchronize()
+
+    def time(self, reps):
+        """ms per call over reps replays, HIP events on the replaying stream."""
+        import torch
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record()
+        for _ in range(reps):
+            self.graph.replay()
+        t1.record()
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / (reps * self.calls)
+
+
+def per_call_ms(fn, steps):
+    import torch
+    for _ in range(10):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def serving_runs(phys, dev, n_requests, rates, seed=7):
+    """Poisson streams of mixed requests through the batching loop and through one
+    call per request."""
+    import numpy as np
+    import torch
+    from jcm_amd import synthetic
+    from jcm_amd.physics_interface import PhysicsTendency
+    from jcm_amd.serving import PhysicsServer, Request
+    res = [(64, 32), (96, 48), (128, 64), (256, 128)]
+    # device-resident member states, one per resolution, reused by the requests
+    pool = []
+    for i, (nx, ny) in enumerate(res):
+        st, fo, ge, da = to_device(synthetic.make_case(nx, ny, 8, seed=100 + i, land=True), dev)
+        cot = PhysicsTendency(*[torch.randn(st.temperature.shape, device=dev) for _ in range(4)])
+        pool.append((st, fo, ge, da, cot))
+    out = {"workload": f"{n_requests} requests, resolution uniform over T21/T31/T42/T85 x 8 levels, "
+                       "50/50 forward and forward+VJP, Poisson arrivals, 2 streams, 4 GiB HBM budget",
+           "runs": []}
+    for rate in rates:
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+        picks = rng.integers(0, len(res), n_requests)
+        kinds = rng.integers(0, 2, n_requests)
+        run = {"rate_per_s": rate}
+        for mode, batching in (("batched", True), ("per_request", False)):
+            reqs = []
+            for i in range(n_requests):
+                st, fo, ge, da, cot = pool[picks[i]]
+                reqs.append(Request(state=st, forcing=fo, geometry=ge, date=da,
+                                    cotangent=cot if kinds[i] else None, arrival=float(arrivals[i]), rid=i))
+            server = PhysicsServer(phys, hbm_budget=4 << 30, n_streams=2, batching=batching)
+            server.run(reqs, realtime=True, timeout=60.0)
+            torch.cuda.synchronize()
+            run[mode] = server.stats(reqs)
+        out["runs"].append(run)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -84,6 +193,7 @@ def main():
         sys.exit(2)
     import torch
     from jcm_amd import _native, synthetic
+    from jcm_amd.physics_interface import PhysicsTendency
     from jcm_amd.speedy_physics import SpeedyPhysics
 
     dev = torch.device("cuda", 0)
@@ -92,34 +202,28 @@ def main():
     kx, ix, il = case_np[2].nodal_shape
     ncol = ix * il
     phys = SpeedyPhysics(device=dev)
-    lib = _native.load()
-
-    def graph_of(calls):
-        stream = torch.cuda.Stream(device=dev)
-        stream.wait_stream(torch.cuda.current_stream(dev))
-        with torch.cuda.stream(stream):
-            phys.compute_tendencies(*case)     # warm the side stream
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=stream):
-            for _ in range(calls):
-                phys.compute_tendencies(*case)
-        torch.cuda.synchronize()
-        return g
+    vphys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2), device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2)
+    cot = PhysicsTendency(*[torch.randn(case[0].temperature.shape, device=dev, generator=gen) for _ in range(4)])
+    fwd = lambda: phys.compute_tendencies(*case)
+    vjp = lambda: vphys.compute_tendencies_vjp(*case, cot)
 
     for _ in range(args.warmup):
-        phys.compute_tendencies(*case)
+        fwd()
+        vjp()
     torch.cuda.synchronize()
     G = max(1, min(args.graph_calls, args.steps))
     reps = max(1, (args.steps + G - 1) // G)
     steps = reps * G
-    graph = graph_of(G)
-    graph.replay()
-    torch.cuda.synchronize()
+    g_fwd = GraphStep(fwd, G, dev)
+    g_vjp = GraphStep(vjp, G, dev)
 
     if args.profile_only:
         for _ in range(reps):
-            graph.replay()
+            g_fwd.graph.replay()
+        for _ in range(reps):
+            g_vjp.graph.replay()
         torch.cuda.synchronize()
         # A copy of known size (torch.neg: reads and writes copy_mib MiB each) whose
         # FETCH_SIZE / WRITE_SIZE counters calibrate the counter units per byte.
@@ -129,47 +233,29 @@ def main():
         for _ in range(3):
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
-        print(f"profile run: {steps} forward calls at T85 replayed from graphs of {G}; "
+        print(f"profile run: {steps} forward and {steps} forward+VJP calls at T85 replayed from graphs of {G}; "
               f"3 x neg of {args.copy_mib} MiB", flush=True)
         return
 
-    # ---- timed region: K steps replayed from graphs, bracketed by synchronize
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_start.record()
-    for _ in range(reps):
-        graph.replay()
-    t_end.record()
-    torch.cuda.synchronize()
-    ms = t_start.elapsed_time(t_end)
-    ms_per_step = ms / steps
+    # ---- timed region (headline): K forward steps replayed from graphs
+    ms_per_step = g_fwd.time(reps)
     value = ncol / (ms_per_step * 1e-3)
+    ms_vjp = g_vjp.time(reps)
 
-    # ---- kernel launches per step (library timer on plain calls, counts only)
+    # ---- launches per call (library timer on plain calls: counts only)
+    lib = _native.load()
     lib.speedy_timer_enable(64)
-    for _ in range(4):
-        phys.compute_tendencies(*case)
+    fwd()
+    vjp()
     torch.cuda.synchronize()
     tm = _native.timer_read()
     lib.speedy_timer_disable()
-    launches = {k: v[1] // 4 for k, v in tm.items() if v[1]}
-    # The dominant (forward) kernel's average launch time inside the timed region:
-    # HIP events around the replays of the same graph, per launch.
-    fwd_ms = ms_per_step if launches.get("levelset", 0) == 0 else None
-    if fwd_ms is None:
-        # two-launch path: time the forward launches alone is not separable from the
-        # graph; fall back to the event timer around each launch on plain calls
-        lib.speedy_timer_enable(2 * 200 + 8)
-        for _ in range(200):
-            phys.compute_tendencies(*case)
-        torch.cuda.synchronize()
-        tm = _native.timer_read()
-        lib.speedy_timer_disable()
-        fwd_ms = tm["fwd"][0] / max(1, tm["fwd"][1])
-    bytes_launch = FWD_BYTES_PER_COL(kx) * ncol
-    achieved = bytes_launch / (fwd_ms * 1e-3) / 1e9
+    launches = {k: v[1] for k, v in tm.items() if v[1]}
 
+    bytes_fwd = FWD_BYTES_PER_COL(kx) * ncol
+    achieved = bytes_fwd / (ms_per_step * 1e-3) / 1e9
+    bytes_vjp = VJP_BYTES_PER_COL(kx) * ncol
+    achieved_vjp = bytes_vjp / (ms_vjp * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "fwd_kernel_T85_traffic.json")
     if os.path.exists(tpath):
@@ -184,9 +270,24 @@ def main():
                    "columns": ncol, "levels": kx, "pass": "fwd", "graph_calls": G},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "speedy::fwd_kernel<8, false, true>", "kernel_ms": fwd_ms,
-                     "launches_per_step": launches, "algorithmic_bytes_per_launch": bytes_launch},
+                     "kernel": "speedy::fwd_kernel<8, false, true, false>", "kernel_ms": ms_per_step,
+                     "algorithmic_bytes_per_launch": bytes_fwd},
+        "fwd_vjp": {"value": ncol / (ms_vjp * 1e-3), "unit": "columns/s", "ms_per_step": ms_vjp,
+                    "launches_per_step": 2,
+                    "roofline": {"bound": "hbm", "achieved": achieved_vjp, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": achieved_vjp / HBM_PEAK_GBS, "algorithmic_bytes_per_step": bytes_vjp,
+                                 "kernel": "speedy::fwd_kernel<8, false, true, true> + param_reduce_kernel"}},
+        "launches_per_call": launches,
     }
+    # ---- plain per-call path (Python, no graphs)
+    pc_f = per_call_ms(fwd, args.per_call_steps)
+    pc_v = per_call_ms(vjp, args.per_call_steps)
+    out["per_call"] = {"fwd_columns_per_s": ncol / (pc_f * 1e-3), "fwd_ms": pc_f,
+                       "fwd_vjp_columns_per_s": ncol / (pc_v * 1e-3), "fwd_vjp_ms": pc_v,
+                       "speedup_graph_over_per_call": {"fwd": pc_f / ms_per_step, "fwd_vjp": pc_v / ms_vjp}}
+    if not args.no_serving:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["serving"] = serving_runs(vphys, dev, args.serve_requests, rates)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case_np, args.cpu_seconds)
     print(json.dumps(out), flush=True)

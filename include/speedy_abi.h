@@ -162,6 +162,42 @@ int speedy_fwd_vjp(const SpeedyFields *fields, const SpeedyAdjoint *adj, const S
                    int32_t nmember, uint32_t flags, int32_t level_set, void *workspace,
                    size_t workspace_bytes, void *stream);
 
+/* One request of a batched call (speedy_fwd_batch): its own grid, members,
+ * fields, date and kind.  All adjoint pointers NULL: forward only; all set:
+ * forward + VJP.  The workspace is the segment's own (speedy_workspace_bytes of
+ * its grid, initialised once by speedy_workspace_init). */
+typedef struct {
+  SpeedyFields fields;          /* fields.diag must be NULL */
+  SpeedyAdjoint adj;
+  SpeedyDate date;
+  int32_t nlon, nlat, nmember;
+  uint32_t flags;
+  int32_t level_set;
+  int32_t reserved;
+  void *workspace;
+  size_t workspace_bytes;
+} SpeedySegment;
+
+#define SPEEDY_BATCH_MAX_SEGMENTS 4096
+
+/* Bytes of the device descriptor buffer (and of the host staging buffer) that
+ * speedy_fwd_batch needs for these segments (0 if they cannot be batched). */
+size_t speedy_batch_desc_bytes(const SpeedySegment *segs, int32_t nseg);
+
+/* Batched forward / forward+VJP of independent requests that share the vertical
+ * grid and the parameters (members of one forecast system at mixed resolutions
+ * and dates): every segment's result equals its own speedy_fwd / speedy_fwd_vjp
+ * call.  No counterpart in the reference (its calls are one grid at a time); it
+ * serves continuous batching.  Writes the descriptor to host_staging (pinned
+ * host memory of speedy_batch_desc_bytes, not to be reused until this call's
+ * copy has completed on `stream`), copies it to `desc` (device memory of the
+ * same size) and launches, in stream order: the level sets of every segment,
+ * one column kernel over all segments' columns, and the parameter-gradient sums
+ * of the VJP segments.  No allocation. */
+int speedy_fwd_batch(const SpeedySegment *segs, int32_t nseg, const SpeedyVertical *vert,
+                     const SpeedyParams *params, void *desc, size_t desc_bytes, void *host_staging,
+                     void *stream);
+
 /* Level set S of the last call's members (bit k = level k), copied to host. */
 int speedy_read_level_sets(const void *workspace, size_t workspace_bytes, int32_t nmember,
                            uint32_t *out, void *stream);

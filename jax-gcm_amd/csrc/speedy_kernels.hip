@@ -239,9 +239,8 @@ __device__ uint32_t collect_chunks(const KArgs& a, uint32_t m) {
 // ---------------------------------------------------------------- kernels
 // Two-kernel path, launch 1: every column's cloud top, ORed per (chunk, member).
 template <int KX>
-__global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
-  extern __shared__ uint32_t s_or[];   // jmax words
-  const uint32_t c_begin = blockIdx.x * a.chunk;
+__device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint32_t* s_or) {
+  const uint32_t c_begin = bid * a.chunk;
   const uint32_t c_end = min(c_begin + a.chunk, a.ntot);
   const uint32_t m_first = divq(c_begin, a.dcol);
   const uint32_t nm = divq(c_end - 1u, a.dcol) - m_first + 1u;
@@ -263,7 +262,13 @@ __global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
     atomicOr(&s_or[m - m_first], clip_bit(cloud_top<float, KX>(col, a.p, a.v, a.u), KX));
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[blockIdx.x * a.jmax + i] = s_or[i];
+  for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[bid * a.jmax + i] = s_or[i];
+}
+
+template <int KX>
+__global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
+  extern __shared__ uint32_t s_or[];   // jmax words
+  levelset_body<KX>(a, blockIdx.x, s_or);
 }
 
 // ---------------------------------------------------------------- VJP helpers
@@ -310,8 +315,8 @@ __device__ __forceinline__ float wave_sum(float v) {
 // Parameter gradients of this block: a fixed-order tree sum per member slot
 // (deterministic), written to prm_part[block][slot][5].  Members narrower than
 // a block write per-column values instead (prm_part[column][5]).
-__device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t gid, bool active, uint32_t slot,
-                                                     const float prm[VJP_NPARAM]) {
+__device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t bid, uint32_t gid, bool active,
+                                                     uint32_t slot, const float prm[VJP_NPARAM]) {
   if (a.ncol < (uint32_t)kBlock) {
     if (active)
 #pragma unroll
@@ -332,7 +337,7 @@ __device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t gi
     float acc = s_red[0][threadIdx.x];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; ++w) acc = acc + s_red[w][threadIdx.x];
-    a.prm_part[(size_t)blockIdx.x * 2 * VJP_NPARAM + threadIdx.x] = acc;
+    a.prm_part[(size_t)bid * 2 * VJP_NPARAM + threadIdx.x] = acc;
   }
 }
 
@@ -340,14 +345,14 @@ __device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t gi
 // the split grid barrier (all blocks co-resident); otherwise S comes from
 // levelset_kernel of the same call.
 template <int KX, bool DIAG, bool FUSED, bool VJP>
-__global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
+__device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   __shared__ uint32_t s_or[2], s_s[2], s_tag;
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t gid = bid * kBlock + threadIdx.x;
   const bool active = gid < a.ntot;
   const uint32_t g0 = gid < a.ntot ? gid : a.ntot - 1u;   // inactive lanes shadow a valid column
   const uint32_t m = divq(g0, a.dcol);
   const uint32_t c = g0 - m * a.ncol;
-  const uint32_t blk0 = blockIdx.x * kBlock;
+  const uint32_t blk0 = bid * kBlock;
   const uint32_t m_first = divq(blk0, a.dcol);
   const uint32_t nm = divq(min(blk0 + kBlock, a.ntot) - 1u, a.dcol) - m_first + 1u;   // <= 2 when fused
   ColumnIn<float, KX> col;
@@ -366,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
     __syncthreads();
     if (threadIdx.x == 0)
       for (uint32_t j = 0; j < nm; ++j)
-        __hip_atomic_store(&a.partial[2u * blockIdx.x + j], (s_tag << 16) | s_or[j], __ATOMIC_RELAXED,
+        __hip_atomic_store(&a.partial[2u * bid + j], (s_tag << 16) | s_or[j], __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
   }
   PhaseB<float, KX> pb;
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
         const uint32_t v = FUSED ? collect_fused(a, s_tag, m_first + j) : collect_chunks(a, m_first + j);
         if (threadIdx.x == 0) s_s[j] = v;
       }
-      if (blockIdx.x == 0) {
+      if (bid == 0) {
         // every member's S for speedy_read_level_sets; in the fused path this also
         // waits for every block's word, i.e. every block has read the generation
         for (uint32_t mm = 0; mm < a.nmember; ++mm) {
@@ -404,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
   load_zonal<KX>(a, c, col);
   phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
   if (active) store_outputs<KX, DIAG>(a, m, c, col, pa, pb);
-  if (VJP) {
+  if (VJP && a.adj.g_t != nullptr) {   // batches mix forward-only and VJP segments
     ColumnCt<float, KX> ct;
     load_cotangents<KX>(a, m, c, ct);
     ColumnAdj<float, KX> g;
@@ -413,14 +418,18 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
     LdsStore tape{s_tape + threadIdx.x};
     column_vjp<float, KX>(col, pa, pb, S, a.p, a.v, a.u, a.flags, ct, g, tape);
     if (active) store_grads<KX>(a, m, c, g);
-    block_param_partials(a, gid, active, m - m_first, g.prm);
+    block_param_partials(a, bid, gid, active, m - m_first, g.prm);
   }
+}
+
+template <int KX, bool DIAG, bool FUSED, bool VJP>
+__global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
+  fwd_body<KX, DIAG, FUSED, VJP>(a, blockIdx.x);
 }
 
 // Per-member parameter gradients from the partials, one block per member, in a
 // fixed order (deterministic).
-__global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) {
-  const uint32_t m = blockIdx.x;
+__device__ __forceinline__ void param_reduce_body(const KArgs& a, uint32_t m) {
   float acc[VJP_NPARAM];
 #pragma unroll
   for (int i = 0; i < VJP_NPARAM; ++i) acc[i] = 0.0f;
@@ -450,6 +459,32 @@ __global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) {
     for (int w = 1; w < kBlock / 64; ++w) t = t + s_red[w][threadIdx.x];
     a.adj.g_params[(size_t)m * VJP_NPARAM + threadIdx.x] = t;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) { param_reduce_body(a, blockIdx.x); }
+
+// ---------------------------------------------------------------- batches
+// A batch descriptor holds one KArgs per segment and, per launch, a table of
+// (segment << 20 | block-in-segment) words; every block reads its word once and
+// then works exactly as in the single-call kernels.
+constexpr uint32_t kSegShift = 20, kSegLocalMask = (1u << kSegShift) - 1u;
+
+template <int KX>
+__global__ __launch_bounds__(kBlock) void levelset_batch_kernel(const KArgs* segs, const uint32_t* table) {
+  extern __shared__ uint32_t s_or[];
+  const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
+  levelset_body<KX>(segs[e >> kSegShift], e & kSegLocalMask, s_or);
+}
+
+template <int KX>
+__global__ __launch_bounds__(kBlock) void fwd_batch_kernel(const KArgs* segs, const uint32_t* table) {
+  const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
+  fwd_body<KX, false, false, true>(segs[e >> kSegShift], e & kSegLocalMask);
+}
+
+__global__ __launch_bounds__(kBlock) void param_reduce_batch_kernel(const KArgs* segs, const uint32_t* table) {
+  const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
+  param_reduce_body(segs[e >> kSegShift], e & kSegLocalMask);
 }
 
 __global__ void fill_zero_kernel(uint32_t* p, int64_t n) {
@@ -644,6 +679,32 @@ int launch_fwd_kx(const KArgs& a, hipStream_t s) {
   return a.f.diag ? launch_fwd<KX, true, false>(a, s) : launch_fwd<KX, false, false>(a, s);
 }
 
+struct BatchLayout {
+  size_t blk, ls, red, total;
+};
+
+int batch_layout(const SpeedySegment* segs, int32_t nseg, BatchLayout& L) {
+  size_t nb = 0, nl = 0, nr = 0;
+  for (int32_t i = 0; i < nseg; ++i) {
+    const SpeedySegment& sg = segs[i];
+    if (sg.nlon <= 0 || sg.nlat <= 0 || sg.nmember <= 0 ||
+        (int64_t)sg.nlon * sg.nlat * sg.nmember >= ((int64_t)1 << 31) - kBlock)
+      return fail(SPEEDY_ERR_ARG, "speedy_fwd_batch: bad grid in segment " + std::to_string(i));
+    const Grid g = make_grid(sg.nlon, sg.nlat, sg.nmember);
+    if (g.nblk > kSegLocalMask || g.ls_blocks > kSegLocalMask)
+      return fail(SPEEDY_ERR_CAPACITY, "speedy_fwd_batch: segment " + std::to_string(i) + " too large for a batch");
+    nb += g.nblk;
+    if (sg.level_set < 0) nl += g.ls_blocks;
+    if (sg.adj.g_t) nr += (size_t)sg.nmember;
+  }
+  if (nb > 0x7fffffffu) return fail(SPEEDY_ERR_CAPACITY, "speedy_fwd_batch: too many blocks");
+  L.blk = align256(sizeof(KArgs) * (size_t)nseg);
+  L.ls = L.blk + align256(sizeof(uint32_t) * nb);
+  L.red = L.ls + align256(sizeof(uint32_t) * nl);
+  L.total = L.red + align256(sizeof(uint32_t) * (nr ? nr : 1));
+  return SPEEDY_OK;
+}
+
 }  // namespace speedy
 
 using namespace speedy;
@@ -734,6 +795,86 @@ int speedy_fwd_vjp(const SpeedyFields* fields, const SpeedyAdjoint* adj, const S
     case 8: return launch_fwd<8, false, true>(a, s);
     default: return launch_fwd<16, false, true>(a, s);
   }
+}
+
+int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertical* vert,
+                     const SpeedyParams* params, void* desc, size_t desc_bytes, void* host_staging, void* stream) {
+  if (!segs || nseg <= 0 || nseg > SPEEDY_BATCH_MAX_SEGMENTS || !vert || !params)
+    return fail(SPEEDY_ERR_ARG, "speedy_fwd_batch: bad segment list");
+  if (vert->kx != 7 && vert->kx != 8 && vert->kx != 16)
+    return fail(SPEEDY_ERR_KX, "speedy_fwd_batch: kx must be 7, 8 or 16");
+  BatchLayout L;
+  if (int rc = batch_layout(segs, nseg, L)) return rc;
+  if (!desc || !host_staging || desc_bytes < L.total)
+    return fail(SPEEDY_ERR_WORKSPACE, "speedy_fwd_batch: descriptor buffers missing or too small (need " +
+                                          std::to_string(L.total) + " bytes)");
+  char* h = static_cast<char*>(host_staging);
+  KArgs* ha = reinterpret_cast<KArgs*>(h);
+  uint32_t* t_blk = reinterpret_cast<uint32_t*>(h + L.blk);
+  uint32_t* t_ls = reinterpret_cast<uint32_t*>(h + L.ls);
+  uint32_t* t_red = reinterpret_cast<uint32_t*>(h + L.red);
+  uint32_t nb = 0, nl = 0, nr = 0, jmax = 1;
+  for (int32_t i = 0; i < nseg; ++i) {
+    const SpeedySegment& sg = segs[i];
+    if (sg.fields.diag) return fail(SPEEDY_ERR_ARG, "speedy_fwd_batch: segments write no diagnostics");
+    const void* ap[] = {sg.adj.ct_du, sg.adj.ct_dv, sg.adj.ct_dt, sg.adj.ct_dq, sg.adj.g_u,
+                        sg.adj.g_v, sg.adj.g_t, sg.adj.g_q, sg.adj.g_ps, sg.adj.g_params};
+    int nset = 0;
+    for (const void* q : ap) nset += q != nullptr;
+    if (nset != 0 && nset != 10)
+      return fail(SPEEDY_ERR_ARG, "speedy_fwd_batch: segment " + std::to_string(i) + " sets only some adjoint pointers");
+    KArgs a;
+    if (int rc = prepare(&sg.fields, vert, params, &sg.date, sg.nlon, sg.nlat, sg.nmember, sg.flags, SPEEDY_TERMS_ALL,
+                         sg.level_set, sg.workspace, sg.workspace_bytes, a))
+      return rc;
+    a.adj = sg.adj;
+    std::memcpy(&ha[i], &a, sizeof(KArgs));
+    for (uint32_t b = 0; b < a.nblk; ++b) t_blk[nb++] = ((uint32_t)i << kSegShift) | b;
+    if (a.level_set < 0) {
+      for (uint32_t b = 0; b < a.ls_blocks; ++b) t_ls[nl++] = ((uint32_t)i << kSegShift) | b;
+      jmax = std::max(jmax, a.jmax);
+    }
+    if (nset) for (uint32_t m = 0; m < a.nmember; ++m) t_red[nr++] = ((uint32_t)i << kSegShift) | m;
+  }
+  g_last_error.clear();
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(desc, host_staging, L.total, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_batch: copy failed: ") + hipGetErrorString(e));
+  const KArgs* d_args = static_cast<const KArgs*>(desc);
+  const char* d = static_cast<const char*>(desc);
+  const uint32_t* d_blk = reinterpret_cast<const uint32_t*>(d + L.blk);
+  const uint32_t* d_ls = reinterpret_cast<const uint32_t*>(d + L.ls);
+  const uint32_t* d_red = reinterpret_cast<const uint32_t*>(d + L.red);
+  auto launch = [&](auto ls_kernel, auto fwd_kernel_b) {
+    if (nl) {
+      const int slot = timer_begin(s, 0);
+      hipLaunchKernelGGL(ls_kernel, dim3(nl), dim3(kBlock), jmax * sizeof(uint32_t), s, d_args, d_ls);
+      timer_end(s, slot);
+    }
+    const int slot = timer_begin(s, 2);
+    hipLaunchKernelGGL(fwd_kernel_b, dim3(nb), dim3(kBlock), 0, s, d_args, d_blk);
+    timer_end(s, slot);
+    if (nr) {
+      const int slot2 = timer_begin(s, 3);
+      hipLaunchKernelGGL(param_reduce_batch_kernel, dim3(nr), dim3(kBlock), 0, s, d_args, d_red);
+      timer_end(s, slot2);
+    }
+  };
+  switch (vert->kx) {
+    case 7: launch(levelset_batch_kernel<7>, fwd_batch_kernel<7>); break;
+    case 8: launch(levelset_batch_kernel<8>, fwd_batch_kernel<8>); break;
+    default: launch(levelset_batch_kernel<16>, fwd_batch_kernel<16>); break;
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_batch: launch failed: ") + hipGetErrorString(e));
+  return SPEEDY_OK;
+}
+
+size_t speedy_batch_desc_bytes(const SpeedySegment* segs, int32_t nseg) {
+  if (!segs || nseg <= 0 || nseg > SPEEDY_BATCH_MAX_SEGMENTS) return 0;
+  BatchLayout L;
+  if (batch_layout(segs, nseg, L)) return 0;
+  return L.total;
 }
 
 int speedy_read_level_sets(const void* workspace, size_t workspace_bytes, int32_t nmember, uint32_t* out,

@@ -67,6 +67,16 @@ class SpeedyAdjoint(ctypes.Structure):
     _fields_ = [(n, CFP) for n in ADJ_NAMES]
 
 
+class SpeedySegment(ctypes.Structure):
+    _fields_ = [("fields", SpeedyFields), ("adj", SpeedyAdjoint), ("date", SpeedyDate),
+                ("nlon", ctypes.c_int32), ("nlat", ctypes.c_int32), ("nmember", ctypes.c_int32),
+                ("flags", ctypes.c_uint32), ("level_set", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("workspace", CFP), ("workspace_bytes", ctypes.c_size_t)]
+
+
+BATCH_MAX_SEGMENTS = 4096
+
+
 class SpeedyError(RuntimeError):
     pass
 
@@ -104,6 +114,11 @@ def load(path: str | None = None):
                                    ctypes.POINTER(SpeedyVertical), ctypes.POINTER(SpeedyParams),
                                    ctypes.POINTER(SpeedyDate), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.c_uint32, ctypes.c_int32, CFP, ctypes.c_size_t, CFP]
+    lib.speedy_batch_desc_bytes.restype = ctypes.c_size_t
+    lib.speedy_batch_desc_bytes.argtypes = [ctypes.POINTER(SpeedySegment), ctypes.c_int32]
+    lib.speedy_fwd_batch.restype = ctypes.c_int
+    lib.speedy_fwd_batch.argtypes = [ctypes.POINTER(SpeedySegment), ctypes.c_int32, ctypes.POINTER(SpeedyVertical),
+                                     ctypes.POINTER(SpeedyParams), CFP, ctypes.c_size_t, CFP, CFP]
     lib.speedy_read_level_sets.restype = ctypes.c_int
     lib.speedy_read_level_sets.argtypes = [CFP, ctypes.c_size_t, ctypes.c_int32,
                                            ctypes.POINTER(ctypes.c_uint32), CFP]

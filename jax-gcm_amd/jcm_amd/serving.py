@@ -1,0 +1,306 @@
+"""Continuous-batching request loop for SPEEDY physics on one GPU.
+
+A request is one forecast member's physics evaluation: its device-resident
+state (torch tensors on the GPU, owned by the client between requests), its
+forcing, geometry and date, and the kind -- forward (tendencies), or forward +
+VJP when a cotangent is given.  Members may have different resolutions (T21,
+T31, T42, T85, ...).  The loop:
+
+  * admits waiting requests in arrival order while their HBM footprint (the
+    outputs plus the call workspace; states are already resident) fits in the
+    budget beside every request on the device, and while the batch stays under
+    a column and a segment limit;
+  * launches each admitted group as ONE batched call (``speedy_fwd_batch``:
+    level sets, one column kernel over all of the batch's columns, parameter
+    sums) on one of ``n_streams`` HIP streams, so a batch can start while the
+    previous one still runs and their work overlaps on the device;
+  * completes a request when its batch's event has fired, releasing its budget
+    so that waiting requests are admitted into the next batch.
+
+Latency is measured on the host clock from the request's arrival to the
+moment the loop observes its batch's completion.  ``batching=False`` runs the
+same stream of requests with one call per request (the plain per-call path) for
+comparison.
+
+No counterpart in the reference (jcm calls the physics once per model step).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Any, Callable, Optional
+
+import numpy as np
+import torch
+
+from . import _native
+from .physics_interface import PhysicsState, PhysicsTendency
+from .speedy_physics import SpeedyPhysics
+
+KIND_FWD = "fwd"
+KIND_VJP = "fwd_vjp"
+
+
+@dataclass
+class Request:
+    """One member's physics call.  Results are filled in when it completes."""
+    state: PhysicsState
+    forcing: Any
+    geometry: Any
+    date: Any
+    cotangent: Optional[PhysicsTendency] = None
+    arrival: float = 0.0          # seconds on the server clock
+    rid: int = 0
+    # filled by the server
+    tendency: Optional[PhysicsTendency] = None
+    state_grads: Optional[PhysicsState] = None
+    param_grads: Optional[dict] = None
+    level_set: Optional[int] = None
+    t_admit: Optional[float] = None
+    t_done: Optional[float] = None
+    batch: int = -1
+
+    @property
+    def kind(self) -> str:
+        return KIND_VJP if self.cotangent is not None else KIND_FWD
+
+    @property
+    def columns(self) -> int:
+        kx, ix, il = self.geometry.nodal_shape
+        t = self.state.temperature
+        nmember = 1 if t.dim() == 3 else int(np.prod(t.shape[:-3]))
+        return ix * il * nmember
+
+    @property
+    def latency(self) -> Optional[float]:
+        return None if self.t_done is None else self.t_done - self.arrival
+
+
+def request_footprint(req: Request, kx: int, workspace_bytes: int) -> int:
+    """HBM bytes a request holds while it is on the device: its outputs
+    (tendencies; for the VJP also state and parameter gradients) and its call
+    workspace.  States and cotangents are resident before admission."""
+    ix, il = req.geometry.nodal_shape[1:]
+    nmember = req.columns // (ix * il)
+    n3 = kx * req.columns * 4                          # one 3-D float field of the request
+    out = 4 * n3                                       # du, dv, dT, dq
+    if req.kind == KIND_VJP:
+        out += 4 * n3 + req.columns * 4 + nmember * 5 * 4   # dL/d(u, v, T, q), dL/dps, parameter sums
+    return out + workspace_bytes
+
+
+@dataclass
+class _Slot:
+    stream: Any
+    desc: Optional[torch.Tensor] = None       # device descriptor buffer
+    staging: Optional[torch.Tensor] = None    # pinned host staging
+    event: Any = None
+    batch: list = field(default_factory=list)
+    keep: list = field(default_factory=list)
+    busy: bool = False
+
+
+class PhysicsServer:
+    """Continuous-batching server over one GPU (see the module docstring)."""
+
+    def __init__(self, physics: SpeedyPhysics | None = None, hbm_budget: int = 8 << 30,
+                 max_batch_columns: int = 1 << 20, max_segments: int = 64, n_streams: int = 2,
+                 batching: bool = True, clock: Callable[[], float] = time.perf_counter):
+        self.phys = physics or SpeedyPhysics()
+        self.dev = self.phys.device
+        self.lib = self.phys._lib
+        self.hbm_budget = int(hbm_budget)
+        self.max_batch_columns = int(max_batch_columns)
+        self.max_segments = min(int(max_segments), _native.BATCH_MAX_SEGMENTS)
+        self.batching = batching
+        self.clock = clock
+        self.slots = [_Slot(stream=torch.cuda.Stream(device=self.dev)) for _ in range(max(1, n_streams))]
+        self.queue: deque = deque()
+        self.in_flight_bytes = 0
+        self.peak_bytes = 0
+        self.done: list = []
+        self.n_batches = 0
+        self.batch_sizes: list = []
+        self._ws_free: dict = {}      # (nlon, nlat, nmember) -> [workspace tensors]
+
+    # ------------------------------------------------------------ resources
+    def _workspace(self, key, stream) -> torch.Tensor:
+        free = self._ws_free.setdefault(key, [])
+        if free:
+            return free.pop()
+        nbytes = self.lib.speedy_workspace_bytes(*key)
+        if nbytes == 0:
+            raise _native.SpeedyError(f"grid {key} is not supported")
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+        _native.check(self.lib.speedy_workspace_init(ws.data_ptr(), nbytes, ctypes.c_void_p(stream.cuda_stream)),
+                      "speedy_workspace_init")
+        return ws
+
+    def footprint(self, req: Request) -> int:
+        kx, ix, il = req.geometry.nodal_shape
+        nmember = req.columns // (ix * il)
+        return request_footprint(req, kx, self.lib.speedy_workspace_bytes(ix, il, nmember))
+
+    # ------------------------------------------------------------ loop
+    def submit(self, req: Request):
+        if self.footprint(req) > self.hbm_budget:
+            raise _native.SpeedyError(f"request {req.rid} needs more HBM than the whole budget")
+        self.queue.append(req)
+
+    def _free_slot(self) -> Optional[_Slot]:
+        for s in self.slots:
+            if not s.busy:
+                return s
+        return None
+
+    def _admit(self) -> list:
+        batch, cols = [], 0
+        limit = self.max_segments if self.batching else 1
+        while self.queue and len(batch) < limit:
+            req = self.queue[0]
+            fp = self.footprint(req)
+            if self.in_flight_bytes + fp > self.hbm_budget:
+                break
+            if batch and cols + req.columns > self.max_batch_columns:
+                break
+            self.queue.popleft()
+            self.in_flight_bytes += fp
+            self.peak_bytes = max(self.peak_bytes, self.in_flight_bytes)
+            req.t_admit = self.clock()
+            batch.append(req)
+            cols += req.columns
+        return batch
+
+    def _launch(self, slot: _Slot, batch: list):
+        phys, lib = self.phys, self.lib
+        keep: list = []
+        segs = (_native.SpeedySegment * len(batch))()
+        s_ptr = ctypes.c_void_p(slot.stream.cuda_stream)
+        # the states were written on the client's stream
+        slot.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(slot.stream):
+            for i, req in enumerate(batch):
+                if req.kind == KIND_VJP:
+                    f, adj, outs, (ix, il, nm), flags = phys.prepare_vjp(req.state, req.forcing, req.geometry,
+                                                                         req.date, req.cotangent, keep)
+                    segs[i].adj = adj
+                    req.state_grads, req.param_grads = outs["state_grads"], outs["param_grads"]
+                else:
+                    f, outs, (ix, il, nm), flags = phys.prepare_fwd(req.state, req.forcing, req.geometry,
+                                                                    req.date, keep)
+                req.tendency = outs["tendency"]
+                ws = self._workspace((ix, il, nm), slot.stream)
+                req._ws = ((ix, il, nm), ws)
+                segs[i].fields = f
+                segs[i].date = req.date.to_abi()
+                segs[i].nlon, segs[i].nlat, segs[i].nmember = ix, il, nm
+                segs[i].flags, segs[i].level_set = flags, -1
+                segs[i].workspace, segs[i].workspace_bytes = ws.data_ptr(), ws.numel()
+            vert = batch[0].geometry.to_abi()
+            for req in batch[1:]:
+                if req.geometry.nodal_shape[0] != batch[0].geometry.nodal_shape[0]:
+                    raise _native.SpeedyError("a batch shares one vertical grid")
+            nbytes = lib.speedy_batch_desc_bytes(segs, len(batch))
+            if nbytes == 0:
+                raise _native.SpeedyError("speedy_batch_desc_bytes failed: " + lib.speedy_last_error().decode())
+            if slot.desc is None or slot.desc.numel() < nbytes:
+                if slot.event is not None:
+                    slot.event.synchronize()
+                cap = max(nbytes, 1 << 16)
+                slot.desc = torch.empty(cap, dtype=torch.uint8, device=self.dev)
+                slot.staging = torch.empty(cap, dtype=torch.uint8, pin_memory=True)
+            rc = lib.speedy_fwd_batch(segs, len(batch), ctypes.byref(vert), ctypes.byref(phys._params()),
+                                      slot.desc.data_ptr(), slot.desc.numel(), slot.staging.data_ptr(), s_ptr)
+            _native.check(rc, "speedy_fwd_batch")
+            ev = torch.cuda.Event()
+            ev.record(slot.stream)
+        slot.event, slot.batch, slot.keep, slot.busy = ev, batch, keep, True
+        for req in batch:
+            req.batch = self.n_batches
+        self.n_batches += 1
+        self.batch_sizes.append(len(batch))
+
+    def _launch_single(self, slot: _Slot, req: Request):
+        """The per-call path: one speedy_fwd / speedy_fwd_vjp call for one request."""
+        slot.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(slot.stream):
+            if req.kind == KIND_VJP:
+                req.tendency, req.state_grads, req.param_grads = self.phys.compute_tendencies_vjp(
+                    req.state, req.forcing, req.geometry, req.date, req.cotangent)
+            else:
+                req.tendency, _ = self.phys.compute_tendencies(req.state, req.forcing, req.geometry, req.date)
+            ev = torch.cuda.Event()
+            ev.record(slot.stream)
+        req._ws = None
+        slot.event, slot.batch, slot.keep, slot.busy = ev, [req], [], True
+        req.batch = self.n_batches
+        self.n_batches += 1
+        self.batch_sizes.append(1)
+
+    def poll(self) -> int:
+        """Complete every batch whose event has fired; returns how many requests completed."""
+        n = 0
+        for slot in self.slots:
+            if slot.busy and slot.event.query():
+                now = self.clock()
+                for req in slot.batch:
+                    req.t_done = now
+                    self.in_flight_bytes -= self.footprint(req)
+                    if req._ws is not None:
+                        key, ws = req._ws
+                        self._ws_free.setdefault(key, []).append(ws)
+                    req._ws = None
+                    self.done.append(req)
+                    n += 1
+                slot.batch, slot.keep, slot.busy = [], [], False
+        return n
+
+    def step(self) -> bool:
+        """Launch one batch if a stream is free and a request can be admitted."""
+        slot = self._free_slot()
+        if slot is None or not self.queue:
+            return False
+        batch = self._admit()
+        if not batch:
+            return False
+        if self.batching:
+            self._launch(slot, batch)
+        else:
+            self._launch_single(slot, batch[0])
+        return True
+
+    def run(self, requests: list, realtime: bool = True, timeout: float = 120.0) -> list:
+        """Serve `requests` (sorted by arrival, seconds relative to the start).  With
+        realtime, a request enters the queue only once the clock reaches its arrival
+        time; otherwise all are queued at once.  Returns the completed requests."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        t0 = self.clock()
+        for r in pending:
+            r.arrival = r.arrival + t0 if realtime else t0
+        while pending or self.queue or any(s.busy for s in self.slots):
+            now = self.clock()
+            while pending and (not realtime or pending[0].arrival <= now):
+                self.submit(pending.popleft())
+            self.poll()
+            while self.step():
+                pass
+            if now - t0 > timeout:
+                raise _native.SpeedyError("serving loop timed out")
+        return self.done
+
+    # ------------------------------------------------------------ report
+    def stats(self, requests: list | None = None) -> dict:
+        reqs = requests if requests is not None else self.done
+        lat = np.array([r.latency for r in reqs if r.latency is not None])
+        if len(lat) == 0:
+            return {}
+        t_first = min(r.arrival for r in reqs)
+        t_last = max(r.t_done for r in reqs)
+        cols = sum(r.columns for r in reqs)
+        return {"requests": len(reqs), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
+                "columns_per_s": cols / max(t_last - t_first, 1e-9), "batches": self.n_batches,
+                "mean_batch": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0,
+                "peak_hbm_bytes": int(self.peak_bytes), "hbm_budget_bytes": self.hbm_budget}

@@ -211,26 +211,23 @@ class SpeedyPhysics(Physics):
         data = PhysicsData(diag, _native.diag_names(kx), kx, (ix, il)) if diag is not None else None
         return PhysicsTendency(*out), data
 
-    def compute_tendencies_vjp(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
-                               date: DateData, cotangent: PhysicsTendency, *, level_set: int = -1):
-        """Fused forward + VJP of compute_tendencies: returns (tendency, state_grads, param_grads)
-        where state_grads is a PhysicsState of gradients (geopotential entry None) and
-        param_grads maps the five VJP parameters (params.VJP_PARAMS) to per-member sums."""
-        self._require_gpu()
-        keep: list = []
+    def prepare_vjp(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry, date: DateData,
+                    cotangent: PhysicsTendency, keep: list):
+        """C structs of one fused forward + VJP call and its freshly allocated outputs:
+        (SpeedyFields, SpeedyAdjoint, outputs, (nlon, nlat, nmember), flags).  `keep`
+        collects the tensors the call reads, which must outlive it."""
         f, t, nmember = self._fields(state, forcing, geometry, date, keep)
         kx, ix, il = geometry.nodal_shape
         out = [torch.empty_like(t) for _ in range(4)]
         f.du, f.dv, f.dt, f.dq = (o.data_ptr() for o in out)
         f.diag = None
         adj = _native.SpeedyAdjoint()
-        cts = []
         for name, src in zip(("ct_du", "ct_dv", "ct_dt", "ct_dq"),
                              ("u_wind", "v_wind", "temperature", "specific_humidity")):
             x = _device(getattr(cotangent, src), self.device)
             if x.shape != t.shape:
                 raise ValueError(f"cotangent {src} has the wrong shape")
-            cts.append(x)
+            keep.append(x)
             setattr(adj, name, x.data_ptr())
         grads = [torch.empty_like(t) for _ in range(4)]
         g_ps = torch.empty(t.shape[:-3] + (ix, il), dtype=torch.float32, device=self.device)
@@ -238,15 +235,36 @@ class SpeedyPhysics(Physics):
         adj.g_u, adj.g_v, adj.g_t, adj.g_q = (g.data_ptr() for g in grads)
         adj.g_ps = g_ps.data_ptr()
         adj.g_params = g_par.data_ptr()
-        ws = self.workspace(ix, il, nmember)
         flags = _native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0
+        outputs = dict(tendency=PhysicsTendency(*out),
+                       state_grads=PhysicsState(grads[0], grads[1], grads[2], grads[3], None, g_ps),
+                       param_grads={f"{g}.{n}": g_par[..., i] for i, (g, n) in enumerate(VJP_PARAMS)})
+        return f, adj, outputs, (ix, il, nmember), flags
+
+    def prepare_fwd(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry, date: DateData, keep: list):
+        """As prepare_vjp for a forward call without diagnostics."""
+        f, t, nmember = self._fields(state, forcing, geometry, date, keep)
+        kx, ix, il = geometry.nodal_shape
+        out = [torch.empty_like(t) for _ in range(4)]
+        f.du, f.dv, f.dt, f.dq = (o.data_ptr() for o in out)
+        f.diag = None
+        flags = _native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0
+        return f, dict(tendency=PhysicsTendency(*out)), (ix, il, nmember), flags
+
+    def compute_tendencies_vjp(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
+                               date: DateData, cotangent: PhysicsTendency, *, level_set: int = -1):
+        """Fused forward + VJP of compute_tendencies: returns (tendency, state_grads, param_grads)
+        where state_grads is a PhysicsState of gradients (geopotential entry None) and
+        param_grads maps the five VJP parameters (params.VJP_PARAMS) to per-member sums."""
+        self._require_gpu()
+        keep: list = []
+        f, adj, outputs, (ix, il, nmember), flags = self.prepare_vjp(state, forcing, geometry, date, cotangent, keep)
+        ws = self.workspace(ix, il, nmember)
         rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(geometry.to_abi()),
                                       ctypes.byref(self._params()), ctypes.byref(date.to_abi()), ix, il, nmember,
                                       flags, level_set, ws.data_ptr(), ws.numel(), self._stream())
         _native.check(rc, "speedy_fwd_vjp")
-        sg = PhysicsState(grads[0], grads[1], grads[2], grads[3], None, g_ps)
-        pg = {f"{g}.{n}": g_par[..., i] for i, (g, n) in enumerate(VJP_PARAMS)}
-        return PhysicsTendency(*out), sg, pg
+        return outputs["tendency"], outputs["state_grads"], outputs["param_grads"]
 
     def level_sets(self, nlon: int, nlat: int, nmember: int = 1) -> list:
         """Q1 level set of each member in the last call on this grid."""

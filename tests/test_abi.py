@@ -27,6 +27,18 @@ def test_library_exports_declared_symbols():
         assert hasattr(lib, n), n
 
 
+def test_batch_descriptor_sizes():
+    lib = _native.load()
+    seg = (_native.SpeedySegment * 2)()
+    for i, (nx, ny) in enumerate(((64, 32), (256, 128))):
+        seg[i].nlon, seg[i].nlat, seg[i].nmember, seg[i].level_set = nx, ny, 1, -1
+    n = lib.speedy_batch_desc_bytes(seg, 2)
+    assert n > 0 and n % 256 == 0
+    seg[1].nlon = 0
+    assert lib.speedy_batch_desc_bytes(seg, 2) == 0
+    assert lib.speedy_batch_desc_bytes(seg, 0) == 0
+
+
 def test_struct_layouts():
     assert ctypes.sizeof(_native.SpeedyParams) == 4 * (59 + 4)
     assert ctypes.sizeof(_native.SpeedyVertical) == 4 * (1 + 17 + 5 * 16 + 32)
