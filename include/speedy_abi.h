@@ -47,6 +47,14 @@ enum {
 
 /* flags */
 #define SPEEDY_FLAG_LFLUXLAND 0x1u   /* ForcingData.lfluxland (forcing.py:17) */
+/* The two steps around the physics in get_physical_tendencies
+ * (physics_interface.py:371-375), fused into the same kernel:
+ * VERIFY_STATE: the physics sees q clamped at 0 (verify_state, :305-319);
+ * VERIFY_TENDENCIES: the q tendency is limited so that q + dt*dq >= 0 with the
+ * unclamped q and dt = SpeedyDate.dt_seconds (verify_tendencies, :321-342). */
+#define SPEEDY_FLAG_VERIFY_STATE 0x2u
+#define SPEEDY_FLAG_VERIFY_TENDENCIES 0x4u
+#define SPEEDY_FLAGS_ALL 0x7u
 
 /* term_mask bit i selects the tendency of term i (speedy_physics.py:67-80)
  * for the sum; every term always runs, so the data handed between terms are

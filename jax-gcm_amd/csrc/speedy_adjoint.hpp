@@ -60,6 +60,25 @@ SPD_HD void adj_zero(ColumnAdj<R, KX>& g) {
   g.rps = R(0.0);
 }
 
+// verify_tendencies (physics_interface.py:321-342) in reverse: where a level's
+// q tendency was limited to -q/dt, its cotangent reaches the unclamped q as
+// -1/dt and none of it reaches the physics.  g_qraw receives that part.
+template <typename R, int KX>
+SPD_HD void verify_ct(uint32_t flags, uint32_t limited, R dt, ColumnCt<R, KX>& ct, R g_qraw[KX]) {
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    g_qraw[k] = R(0.0);
+    if ((flags & SPEEDY_FLAG_VERIFY_TENDENCIES) && ((limited >> k) & 1u)) {
+      g_qraw[k] = -ct.dq[k] / dt;
+      ct.dq[k] = R(0.0);
+    }
+  }
+}
+
+// verify_state in reverse: where(q < 0, 0, q) passes the gradient only where q >= 0.
+template <typename R>
+SPD_HD R verify_q_grad(R q_raw, R g) { return q_raw < R(0.0) ? R(0.0) : g; }
+
 // d qsat / d ta and d qsat / d ps of qsat_f (humidity.py:95-102).
 template <typename R>
 SPD_HD void qsat_grad(R ta, R ps, R sig, R& dq_dt, R& dq_dps) {

@@ -122,6 +122,22 @@ SPD_HD void zonal_column(const ZonalUniforms<R>& z, R sia, R coa, R out[6]) {
   out[5] = msqrt(coa);
 }
 
+// verify_state (physics_interface.py:305-319): where(q < 0, 0, q).
+template <typename R>
+SPD_HD R verify_q(R q) { return q < R(0.0) ? R(0.0) : q; }
+
+// verify_tendencies (physics_interface.py:321-342) for one level, with the
+// unclamped q: where(q + dt*dq >= 0, dq, -q/dt).  Sets `limited`.
+template <typename R>
+SPD_HD R verify_dq(R q, R dq, R dt, bool& limited) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  const R qn = q + dt * dq;
+  limited = !(qn >= R(0.0));
+  return limited ? -q / dt : dq;
+}
+
 // Inputs of one column.
 template <typename R, int KX>
 struct ColumnIn {
@@ -151,6 +167,7 @@ template <typename R>
 struct Uniforms {
   int32_t kx;
   uint32_t compute_sw;            // set_physics_flags: model_step % nstrad == 0 (speedy_physics.py:26-27)
+  R dt_step;                      // DateData.dt_seconds: the time step of verify_tendencies
   R wvi1[SPEEDY_KX_MAX];      // geometry.wvi[:, 1]
   // convection (convection.py:136-142)
   R entr[SPEEDY_KX_MAX], fm0, rdps;
@@ -184,6 +201,7 @@ inline void make_uniforms(const SpeedyParams& p, const SpeedyVertical& v, const 
   int32_t ms = d.model_step % 3;
   if (ms < 0) ms += 3;
   u.compute_sw = ms == 0;
+  u.dt_step = d.dt_seconds;
   for (int k = 0; k < kx; ++k) u.wvi1[k] = v.wvi[k][1];
   R e[SPEEDY_KX_MAX] = {0};
   R sentr = R(0.0);

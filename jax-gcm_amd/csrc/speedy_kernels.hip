@@ -106,6 +106,10 @@ __device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t 
     col.Q[k] = a.f.q[base3 + k * ncol];
     col.PHI[k] = a.f.phi[base3 + k * ncol];
   }
+  if (a.flags & SPEEDY_FLAG_VERIFY_STATE) {
+#pragma unroll
+    for (int k = 0; k < KX; ++k) col.Q[k] = verify_q(col.Q[k]);
+  }
   col.u1 = a.f.u[base3 + (KX - 1) * ncol];
   col.v1 = a.f.v[base3 + (KX - 1) * ncol];
   col.ps = a.f.ps[(size_t)m * ncol + c];
@@ -128,9 +132,12 @@ __device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<
   col.fsol = z[0]; col.ozupp = z[1]; col.ozone = z[2]; col.zenit = z[3]; col.stratz = z[4]; col.sqcoa = z[5];
 }
 
+// Returns the levels whose q tendency verify_tendencies limited (bit k).
 template <int KX, bool DIAG>
-__device__ __forceinline__ void store_outputs(const KArgs& a, uint32_t m, uint32_t c, const ColumnIn<float, KX>& col,
-                                              const PhaseA<float, KX>& pa, const PhaseB<float, KX>& pb) {
+__device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, uint32_t c,
+                                                  const ColumnIn<float, KX>& col, const PhaseA<float, KX>& pa,
+                                                  const PhaseB<float, KX>& pb) {
+  uint32_t limited = 0u;
   // term-ordered sum (speedy_physics.py:123-125)
   const uint32_t tm = a.term_mask;
   const size_t ncol = a.ncol;
@@ -146,6 +153,11 @@ __device__ __forceinline__ void store_outputs(const KArgs& a, uint32_t m, uint32
     }
     if (tm & (1u << SPEEDY_TERM_LW_UP)) t = t + pb.tlw[k];
     if (tm & (1u << SPEEDY_TERM_VDIFF)) { t = t + pb.tvd[k]; q = q + pb.qvd[k]; }
+    if (a.flags & SPEEDY_FLAG_VERIFY_TENDENCIES) {
+      bool lim;
+      q = verify_dq(a.f.q[base3 + k * ncol], q, a.u.dt_step, lim);   // the unclamped q (re-read, L2)
+      limited |= (lim ? 1u : 0u) << k;
+    }
     a.f.du[base3 + k * ncol] = du;
     a.f.dv[base3 + k * ncol] = dv;
     a.f.dt[base3 + k * ncol] = t;
@@ -188,6 +200,7 @@ __device__ __forceinline__ void store_outputs(const KArgs& a, uint32_t m, uint32
 #pragma unroll
       for (int k = 0; k < KX; ++k) W(pb.tauL[k][b]);
   }
+  return limited;
 }
 
 // ---------------------------------------------------------------- level set S
@@ -257,6 +270,7 @@ __device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint
       col.T[k] = a.f.t[base3 + k * ncol];
       col.Q[k] = a.f.q[base3 + k * ncol];
       col.PHI[k] = a.f.phi[base3 + k * ncol];
+      if (a.flags & SPEEDY_FLAG_VERIFY_STATE) col.Q[k] = verify_q(col.Q[k]);
     }
     col.ps = a.f.ps[(size_t)m * ncol + c];
     atomicOr(&s_or[m - m_first], clip_bit(cloud_top<float, KX>(col, a.p, a.v, a.u), KX));
@@ -408,15 +422,25 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   }
   load_zonal<KX>(a, c, col);
   phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
-  if (active) store_outputs<KX, DIAG>(a, m, c, col, pa, pb);
+  uint32_t limited = 0u;
+  if (active) limited = store_outputs<KX, DIAG>(a, m, c, col, pa, pb);
   if (VJP && a.adj.g_t != nullptr) {   // batches mix forward-only and VJP segments
     ColumnCt<float, KX> ct;
     load_cotangents<KX>(a, m, c, ct);
+    float g_qraw[KX];
+    verify_ct<float, KX>(a.flags, limited, a.u.dt_step, ct, g_qraw);
     ColumnAdj<float, KX> g;
     // longwave flux tapes in LDS, [slot][thread]: the kernel runs one block per CU
     __shared__ float s_tape[vjp_tape_floats<KX>() * kBlock];
     LdsStore tape{s_tape + threadIdx.x};
     column_vjp<float, KX>(col, pa, pb, S, a.p, a.v, a.u, a.flags, ct, g, tape);
+    if (a.flags & SPEEDY_FLAG_VERIFY_STATE) {
+      const size_t base3 = (size_t)m * KX * a.ncol + c;
+#pragma unroll
+      for (int k = 0; k < KX; ++k) g.Q[k] = verify_q_grad(a.f.q[base3 + k * (size_t)a.ncol], g.Q[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < KX; ++k) g.Q[k] += g_qraw[k];
     if (active) store_grads<KX>(a, m, c, g);
     block_param_partials(a, bid, gid, active, m - m_first, g.prm);
   }
@@ -585,7 +609,9 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   if (nlon <= 0 || nlat <= 0 || nmember <= 0) return fail(SPEEDY_ERR_ARG, "speedy: nlon, nlat, nmember must be > 0");
   if (vert->kx != 7 && vert->kx != 8 && vert->kx != 16)
     return fail(SPEEDY_ERR_KX, "speedy: kx must be 7, 8 or 16 (got " + std::to_string(vert->kx) + ")");
-  if (flags & ~(uint32_t)SPEEDY_FLAG_LFLUXLAND) return fail(SPEEDY_ERR_ARG, "speedy: unknown flag bits");
+  if (flags & ~(uint32_t)SPEEDY_FLAGS_ALL) return fail(SPEEDY_ERR_ARG, "speedy: unknown flag bits");
+  if ((flags & SPEEDY_FLAG_VERIFY_TENDENCIES) && !(date->dt_seconds > 0.0f))
+    return fail(SPEEDY_ERR_ARG, "speedy: VERIFY_TENDENCIES needs date->dt_seconds > 0");
   if (term_mask & ~(uint32_t)SPEEDY_TERMS_ALL) return fail(SPEEDY_ERR_ARG, "speedy: unknown term bits");
   if (level_set >= (1 << vert->kx)) return fail(SPEEDY_ERR_ARG, "speedy: level_set has bits beyond kx");
   if (int rc = check_fields(fields)) return rc;

@@ -25,6 +25,7 @@ from .params import VJP_PARAMS, Parameters
 from .physics_interface import Physics, PhysicsState, PhysicsTendency
 
 STATE_FIELDS = ("u_wind", "v_wind", "temperature", "specific_humidity", "geopotential")
+VERIFY_FLAGS = _native.FLAG_VERIFY_STATE | _native.FLAG_VERIFY_TENDENCIES
 
 
 def _device(x, dev) -> torch.Tensor:
@@ -184,11 +185,14 @@ class SpeedyPhysics(Physics):
     # ------------------------------------------------------------------ API
     def compute_tendencies(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
                            date: DateData, *, term_mask: int = _native.TERMS_ALL,
-                           level_set: int = -1) -> Tuple[PhysicsTendency, Any]:
+                           level_set: int = -1, verify: bool = False) -> Tuple[PhysicsTendency, Any]:
         """SpeedyPhysics.compute_tendencies (speedy_physics.py:88-127).
 
         term_mask: bit i keeps term i's tendency in the sum (speedy_physics.py:67-80 order);
-        level_set: >= 0 overrides the Q1 cloud-top level set (parity tests)."""
+        level_set: >= 0 overrides the Q1 cloud-top level set (parity tests);
+        verify: also apply verify_state before and verify_tendencies after, with
+        dt = date.dt_seconds, as get_physical_tendencies does (physics_interface.py:371-375),
+        in the same kernel."""
         self._require_gpu()
         keep: list = []
         f, t, nmember = self._fields(state, forcing, geometry, date, keep)
@@ -203,7 +207,7 @@ class SpeedyPhysics(Physics):
         else:
             f.diag = None
         ws = self.workspace(ix, il, nmember)
-        flags = _native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0
+        flags = (_native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0) | (VERIFY_FLAGS if verify else 0)
         rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(geometry.to_abi()), ctypes.byref(self._params()),
                                   ctypes.byref(date.to_abi()), ix, il, nmember, flags, term_mask, level_set,
                                   ws.data_ptr(), ws.numel(), self._stream())
@@ -252,13 +256,16 @@ class SpeedyPhysics(Physics):
         return f, dict(tendency=PhysicsTendency(*out)), (ix, il, nmember), flags
 
     def compute_tendencies_vjp(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
-                               date: DateData, cotangent: PhysicsTendency, *, level_set: int = -1):
+                               date: DateData, cotangent: PhysicsTendency, *, level_set: int = -1,
+                               verify: bool = False):
         """Fused forward + VJP of compute_tendencies: returns (tendency, state_grads, param_grads)
         where state_grads is a PhysicsState of gradients (geopotential entry None) and
-        param_grads maps the five VJP parameters (params.VJP_PARAMS) to per-member sums."""
+        param_grads maps the five VJP parameters (params.VJP_PARAMS) to per-member sums.
+        verify: as in compute_tendencies (the gradients then include the two steps)."""
         self._require_gpu()
         keep: list = []
         f, adj, outputs, (ix, il, nmember), flags = self.prepare_vjp(state, forcing, geometry, date, cotangent, keep)
+        flags |= VERIFY_FLAGS if verify else 0
         ws = self.workspace(ix, il, nmember)
         rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(geometry.to_abi()),
                                       ctypes.byref(self._params()), ctypes.byref(date.to_abi()), ix, il, nmember,

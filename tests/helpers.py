@@ -82,7 +82,7 @@ def build_host_column():
     return lib
 
 
-def run_host_column(case, params, term_mask=(1 << 11) - 1, level_set=-1, dtype=np.float32):
+def run_host_column(case, params, term_mask=(1 << 11) - 1, level_set=-1, dtype=np.float32, extra_flags=0):
     """Run the host build of the kernel's column routine on a single-member case,
     in float32 (the GPU's arithmetic) or float64 (the same routine in double)."""
     import jcm_amd._native as nat
@@ -98,7 +98,7 @@ def run_host_column(case, params, term_mask=(1 << 11) - 1, level_set=-1, dtype=n
     in_arr = (ctypes.c_void_p * len(ins))(*[a.ctypes.data for a in ins])
     out_arr = (ctypes.c_void_p * len(outs))(*[a.ctypes.data for a in outs])
     s_out = ctypes.c_uint32(0)
-    flags = nat.FLAG_LFLUXLAND if forcing.lfluxland else 0
+    flags = (nat.FLAG_LFLUXLAND if forcing.lfluxland else 0) | extra_flags
     fn = lib.column_host_fwd if np.dtype(dtype) == np.float32 else lib.column_host_fwd64
     rc = fn(kx, in_arr, out_arr, ctypes.byref(geometry.to_abi()), ctypes.byref(params.to_abi()),
                              ctypes.byref(date.to_abi()), ix, il, ctypes.c_uint32(flags), ctypes.c_uint32(term_mask),
@@ -128,7 +128,7 @@ def run_oracle64_same_constants(case, params=None, level_set=None, term_mask=so.
     return so.compute_tendencies(st, fo, g, d, p, level_set=level_set, term_mask=term_mask, return_level_set=True)
 
 
-def run_host_vjp(case, params, cot, level_set=-1, dtype=np.float64):
+def run_host_vjp(case, params, cot, level_set=-1, dtype=np.float64, extra_flags=0):
     """Forward + VJP of the host build of the column routine (speedy_adjoint.hpp).
 
     cot: (ct_du, ct_dv, ct_dt, ct_dq), each (kx, ix, il).  Returns (tendencies,
@@ -148,9 +148,23 @@ def run_host_vjp(case, params, cot, level_set=-1, dtype=np.float64):
     gs = [np.zeros((kx, ix, il), dtype) for _ in range(4)] + [np.zeros((ix, il), dtype),
                                                               np.zeros((ix, il, 5), dtype)]
     P = lambda arrs: (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
-    flags = nat.FLAG_LFLUXLAND if forcing.lfluxland else 0
+    flags = (nat.FLAG_LFLUXLAND if forcing.lfluxland else 0) | extra_flags
     fn = lib.column_host_vjp if np.dtype(dtype) == np.float32 else lib.column_host_vjp64
     rc = fn(kx, P(ins), P(cts), P(outs), P(gs), ctypes.byref(geometry.to_abi()), ctypes.byref(params.to_abi()),
             ctypes.byref(date.to_abi()), ix, il, ctypes.c_uint32(flags), ctypes.c_int32(level_set))
     assert rc == 0
     return outs, dict(u=gs[0], v=gs[1], t=gs[2], q=gs[3], ps=gs[4], prm=gs[5])
+
+
+def run_oracle_verified(state, forcing, geometry, date, params=None, dtype=np.float32, level_set=None,
+                        same_constants=False):
+    """get_physical_tendencies' physics part (physics_interface.py:371-375):
+    verify_state -> compute_tendencies -> verify_tendencies(unclamped state, dt)."""
+    if same_constants:
+        st, fo, g, d, p = oracle64_same_constants(state, forcing, geometry, date, params)
+    else:
+        st, fo, g, d, p = oracle_inputs(state, forcing, geometry, date, params, dtype)
+    tend, data, S = so.compute_tendencies(so.verify_state(st), fo, g, d, p, level_set=level_set,
+                                          return_level_set=True)
+    dt = float(np.float32(date.dt_seconds))
+    return so.verify_tendencies(st, tend, dt), data, S

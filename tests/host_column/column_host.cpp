@@ -27,6 +27,7 @@ static void run(const R* const* in, R* const* out, const SpeedyVertical* v, cons
   auto load = [&](int64_t c, ColumnIn<R, KX>& col) {
     for (int k = 0; k < KX; ++k) {
       col.T[k] = t[k * ncol + c]; col.Q[k] = q[k * ncol + c]; col.PHI[k] = phi[k * ncol + c];
+      if (flags & SPEEDY_FLAG_VERIFY_STATE) col.Q[k] = verify_q(col.Q[k]);
     }
     col.u1 = u[(KX - 1) * ncol + c]; col.v1 = vv[(KX - 1) * ncol + c]; col.ps = ps[c];
     col.alb0 = in[6][c]; col.sice = in[7][c]; col.snowc_am = in[8][c]; col.soilw = in[9][c];
@@ -63,6 +64,7 @@ static void run(const R* const* in, R* const* out, const SpeedyVertical* v, cons
       }
       if (term_mask & (1u << SPEEDY_TERM_LW_UP)) tt = tt + b.tlw[k];
       if (term_mask & (1u << SPEEDY_TERM_VDIFF)) { tt = tt + b.tvd[k]; qq = qq + b.qvd[k]; }
+      if (flags & SPEEDY_FLAG_VERIFY_TENDENCIES) { bool lim; qq = verify_dq(q[k * ncol + c], qq, un.dt_step, lim); }
       out[0][k * ncol + c] = du; out[1][k * ncol + c] = dv; out[2][k * ncol + c] = tt; out[3][k * ncol + c] = qq;
     }
     // a few diagnostics: iptop, icltop, cbmf, precnv, precls, cloudc, clstr, rsds, rlds, tskin
@@ -86,6 +88,7 @@ static void run_vjp(const R* const* in, const R* const* ct, R* const* out, R* co
   auto load = [&](int64_t c, ColumnIn<R, KX>& col) {
     for (int k = 0; k < KX; ++k) {
       col.T[k] = t[k * ncol + c]; col.Q[k] = q[k * ncol + c]; col.PHI[k] = phi[k * ncol + c];
+      if (flags & SPEEDY_FLAG_VERIFY_STATE) col.Q[k] = verify_q(col.Q[k]);
     }
     col.u1 = u[(KX - 1) * ncol + c]; col.v1 = vv[(KX - 1) * ncol + c]; col.ps = ps[c];
     col.alb0 = in[6][c]; col.sice = in[7][c]; col.snowc_am = in[8][c]; col.soilw = in[9][c];
@@ -112,6 +115,7 @@ static void run_vjp(const R* const* in, const R* const* ct, R* const* out, R* co
     PhaseB<R, KX> b;
     phase_b1<R, KX>(col, a, *p, *v, un, b);
     phase_b2<R, KX, false>(col, a, S, *p, *v, un, flags, b);
+    uint32_t limited = 0;
     for (int k = 0; k < KX; ++k) {
       R tt = 0, qq = 0, du = 0, dv = 0;
       tt = tt + a.tconv[k]; qq = qq + a.qconv[k];
@@ -120,15 +124,26 @@ static void run_vjp(const R* const* in, const R* const* ct, R* const* out, R* co
       if (k == KX - 1) { du = du + b.du1; dv = dv + b.dv1; tt = tt + b.dts1; qq = qq + b.dqs1; }
       tt = tt + b.tlw[k];
       tt = tt + b.tvd[k]; qq = qq + b.qvd[k];
+      if (flags & SPEEDY_FLAG_VERIFY_TENDENCIES) {
+        bool lim;
+        qq = verify_dq(q[k * ncol + c], qq, un.dt_step, lim);
+        limited |= (lim ? 1u : 0u) << k;
+      }
       out[0][k * ncol + c] = du; out[1][k * ncol + c] = dv; out[2][k * ncol + c] = tt; out[3][k * ncol + c] = qq;
     }
     ColumnCt<R, KX> cc;
     cc.du1 = ct[0][(KX - 1) * ncol + c];
     cc.dv1 = ct[1][(KX - 1) * ncol + c];
     for (int k = 0; k < KX; ++k) { cc.dt[k] = ct[2][k * ncol + c]; cc.dq[k] = ct[3][k * ncol + c]; }
+    R g_qraw[KX];
+    verify_ct<R, KX>(flags, limited, un.dt_step, cc, g_qraw);
     ColumnAdj<R, KX> ga;
     ArrayStore<R, vjp_tape_floats<KX>()> tape;
     column_vjp<R, KX>(col, a, b, S, *p, *v, un, flags, cc, ga, tape);
+    for (int k = 0; k < KX; ++k) {
+      if (flags & SPEEDY_FLAG_VERIFY_STATE) ga.Q[k] = verify_q_grad(q[k * ncol + c], ga.Q[k]);
+      ga.Q[k] += g_qraw[k];
+    }
     for (int k = 0; k < KX; ++k) {
       g[0][k * ncol + c] = k == KX - 1 ? ga.u1 : R(0);
       g[1][k * ncol + c] = k == KX - 1 ? ga.v1 : R(0);

@@ -205,3 +205,37 @@ def test_adjoint_configuration_t42_parameter_gradients_h1e3():
         assert (~ok).sum() <= 0.02 * ix * il
         assert np.all(np.abs(per_col[ok] - f3[ok]) <= 2e-3 * np.abs(f3[ok]) + 1e-6 * scale), nm
         assert abs(ours - ref) <= 1e-4 * max(abs(ref), 1e-12), nm
+
+
+def test_state_gradients_with_fused_verify_steps():
+    """The VJP with verify_state / verify_tendencies fused: central differences of
+    the float64 column routine with the same flags, per field and level."""
+    import jcm_amd._native as nat
+    from test_host_column import _negative_q_case
+    flags = nat.FLAG_VERIFY_STATE | nat.FLAG_VERIFY_TENDENCIES
+    case = _negative_q_case()
+    params = Parameters.default()
+    kx, ix, il = case[2].nodal_shape
+    rng = np.random.default_rng(5)
+    cot = [rng.standard_normal((kx, ix, il)) for _ in range(4)]
+    _, _, S = H.run_host_column(case, params, dtype=np.float64, extra_flags=flags)
+    _, g = H.run_host_vjp(case, params, cot, level_set=S, extra_flags=flags)
+
+    def fd(field, k, h):
+        tp, _, _ = H.run_host_column(_perturbed(case, field, k, h), params, dtype=np.float64, level_set=S,
+                                     extra_flags=flags)
+        tm, _, _ = H.run_host_column(_perturbed(case, field, k, -h), params, dtype=np.float64, level_set=S,
+                                     extra_flags=flags)
+        return (_J(cot, tp) - _J(cot, tm)) / (2 * h)
+
+    for field, levels, h in (("t", range(kx), 1e-4), ("q", range(kx), 1e-7), ("ps", [0], 1e-7)):
+        for k in levels:
+            f1, f2 = fd(field, k, h), fd(field, k, h / 8)
+            smooth = np.abs(f1 - f2) <= 1e-6 * (np.abs(f1) + np.abs(f1).max())
+            grad = g["ps"] if field == "ps" else g[field][k]
+            scale = np.abs(f2).max()
+            if scale == 0:
+                continue
+            err = np.where(smooth, np.abs(grad - f2), 0.0).max() / scale
+            assert err <= FD_TOL, f"{field}[{k}]: {err:.2e}"
+            assert (~smooth).sum() <= 0.02 * ix * il

@@ -303,3 +303,22 @@ def test_fwd_two_kernel_level_set_path(nlon, nlat, nmember):
         for f in FIELDS:
             ok, e32, e64, eref = H.parity(_np(getattr(tend, f)[m]), getattr(ref, f), getattr(ref64, f))
             assert ok, f"member {m} {f}: {e32:.3e} (fp64 {e64:.2e}, oracle {eref:.2e})"
+
+
+def test_fwd_fused_verify_steps(phys):
+    """verify_state -> compute_tendencies -> verify_tendencies (physics_interface.py:371-375)
+    fused into the kernel, against the oracle's sequence; negative humidities and a
+    one-hour step so that both steps act."""
+    import torch
+    from test_host_column import _negative_q_case
+    case = _negative_q_case()
+    phys.parameters = Parameters.default()
+    tend, _ = phys.compute_tendencies(*case, verify=True)
+    torch.cuda.synchronize()
+    ref, _, S_ref = H.run_oracle_verified(*case)
+    ref64, _, _ = H.run_oracle_verified(*case, dtype=np.float64)
+    assert phys.level_sets(64, 32, 1)[0] == S_ref
+    for f in FIELDS:
+        ok, e32, e64, eref = H.parity(_np(getattr(tend, f)), getattr(ref, f), getattr(ref64, f))
+        print(f"verify {f}: {e32:.2e} (fp64 {e64:.2e})", flush=True)
+        assert ok, f"{f}: {e32:.3e}"

@@ -175,3 +175,23 @@ def test_vjp_members_stacked(phys, nlon, nlat, nmember):
         assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
         _, g64 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m])
         _check_against_host({k: v[m] for k, v in g.items()}, prm[m], g64, f"member {m}")
+
+
+def test_vjp_with_fused_verify_steps(phys):
+    import jcm_amd._native as nat
+    import torch
+    from test_host_column import _negative_q_case
+    case = _negative_q_case()
+    params = Parameters.default()
+    kx, ix, il = case[2].nodal_shape
+    cot = _cot((kx, ix, il), 6)
+    phys.parameters = params
+    tend, sg, pg = phys.compute_tendencies_vjp(*case, PhysicsTendency(*cot), verify=True)
+    torch.cuda.synchronize()
+    g = dict(u=_np(sg.u_wind), v=_np(sg.v_wind), t=_np(sg.temperature), q=_np(sg.specific_humidity),
+             ps=_np(sg.normalized_surface_pressure))
+    prm = np.stack([_np(pg[f"{grp}.{nm}"]) for grp, nm in VJP_PARAMS], axis=-1)
+    S = phys.level_sets(ix, il, 1)[0]
+    _, g64 = H.run_host_vjp(case, params, cot, level_set=S,
+                            extra_flags=nat.FLAG_VERIFY_STATE | nat.FLAG_VERIFY_TENDENCIES)
+    _check_against_host(g, prm, g64, "verify")

@@ -139,3 +139,38 @@ def test_host_column_float64_single_terms(term):
             assert np.max(np.abs(a)) == 0, f
         else:
             assert H.rel_inf(a, ref) <= F64_TOL, f
+
+
+def _negative_q_case(seed=3, frac=0.05):
+    """T21 wet land case with a fraction of negative humidities (as the dynamics can
+    leave them) and a long time step, so both verify steps act."""
+    st, fo, ge, da = make("T21_wet", seed)
+    rng = np.random.default_rng(seed)
+    q = st.specific_humidity.copy()
+    neg = rng.random(q.shape) < frac
+    q[neg] = -np.abs(q[neg]) * np.float32(0.05)
+    return st.copy(specific_humidity=q), fo, ge, da.copy(dt_seconds=3600.0)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, H.REL_TOL), (np.float64, F64_TOL)])
+def test_host_column_fused_verify_steps(dtype, tol):
+    """verify_state and verify_tendencies fused into the column routine equal the
+    oracle's verify_state -> compute_tendencies -> verify_tendencies sequence."""
+    import jcm_amd._native as nat
+    case = _negative_q_case()
+    p = Parameters.default()
+    flags = nat.FLAG_VERIFY_STATE | nat.FLAG_VERIFY_TENDENCIES
+    outs, _, S = H.run_host_column(case, p, dtype=dtype, extra_flags=flags)
+    tend, _, S_ref = H.run_oracle_verified(*case, params=p, dtype=dtype, same_constants=(dtype == np.float64))
+    assert S == S_ref
+    q = case[0].specific_humidity
+    lim = q + 3600.0 * tend.specific_humidity.astype(np.float64) < 0
+    assert (q < 0).any() and np.any(tend.specific_humidity == -q / np.float32(3600.0))
+    for f, a in zip(FIELDS, outs):
+        ref = getattr(tend, f)
+        if dtype == np.float32:
+            ref64 = H.run_oracle_verified(*case, params=p, dtype=np.float64)[0]
+            ok, e32, e64, eref = H.parity(a, ref, getattr(ref64, f))
+            assert ok, f"{f}: {e32:.2e}"
+        else:
+            assert H.rel_inf(a, ref) <= tol, f
