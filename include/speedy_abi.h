@@ -112,7 +112,24 @@ typedef struct {
   float *du, *dv, *dt, *dq;
   /* optional PhysicsData subset, [nmember][speedy_diag_rows(kx)][ncol], or NULL */
   float *diag;
+  /* optional float64 budgets, [nmember][SPEEDY_NBUDGET], or NULL (see below) */
+  double *budget;
 } SpeedyFields;
+
+/* Float64 column budgets, area-weighted (weight cos(latitude) = coa) global
+ * means per member, of the call's tendencies (after any verify step):
+ *   0  column energy tendency   sum_k cp dT_k dp_k/g              [W m-2]
+ *   1  column water tendency    sum_k dq_k dp_k/g                 [g m-2 s-1]
+ *   2  surface evaporation      E                                 [g m-2 s-1]
+ *   3  precipitation            P = precnv + precls               [g m-2 s-1]
+ *   4  water budget residual    (column water tendency) - (E - P) [g m-2 s-1]
+ *   5  largest |water residual| of any column                     [g m-2 s-1]
+ * with dp_k/g = dhs_k ps p0 / g.  The per-column sums and the global
+ * reduction run in float64 on the GPU (fixed order: deterministic); the
+ * physics itself stays float32.  Without lfluxland, E is 0.
+ * (Column water conservation: large_scale_condensation.py:77-79,
+ * convection.py:181-219, surface_flux.py:297, vertical_diffusion.py:117-118.) */
+#define SPEEDY_NBUDGET 6
 
 /* Cotangents and gradients of the fused forward+VJP call. */
 typedef struct {

@@ -616,6 +616,7 @@ struct PhaseB {
   // b2
   R tsw[KX], tlw[KX];
   R du1, dv1, dts1, dqs1;        // surface-flux tendencies at level KX-1
+  R evap2;                       // grid-box surface evaporation (budgets)
   R flux_h[2], rlus2;            // shortwave -> longwave and surface -> longwave hand-offs (and tsfc)
   // diagnostics (b2)
   R dfabs_sw[KX], rsds, rsns, ftop_sw, stratc0, stratc1;
@@ -1001,6 +1002,7 @@ SPD_HD void surface(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const Spee
   b.dv1 = vstr2 * rps * v.grdsig[KX - 1];
   b.dts1 = shf2 * rps * v.grdscp[KX - 1];
   b.dqs1 = evap2 * rps * v.grdsig[KX - 1];
+  b.evap2 = evap2;
   if (DIAG) {
     b.ustr[0] = ustr0; b.ustr[1] = ustr1; b.ustr[2] = ustr2;
     b.vstr[0] = vstr0; b.vstr[1] = vstr1; b.vstr[2] = vstr2;

@@ -59,7 +59,10 @@ struct KArgs {
   // fused forward + VJP only
   SpeedyAdjoint adj;
   float* prm_part;                   // parameter-gradient partials (per block and member slot, or per column)
+  double* bud_part;                  // float64 budget partials (same layout, kBudParts doubles each)
 };
+
+constexpr int kBudParts = SPEEDY_NBUDGET + 1;   // the budgets and the weight sum
 
 // ---------------------------------------------------------------- diagnostics
 // Names follow Physics.data_struct_to_dict (physics_interface.py:157-194):
@@ -136,8 +139,12 @@ __device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<
 template <int KX, bool DIAG>
 __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, uint32_t c,
                                                   const ColumnIn<float, KX>& col, const PhaseA<float, KX>& pa,
-                                                  const PhaseB<float, KX>& pb) {
+                                                  const PhaseB<float, KX>& pb, double colsum[2]) {
   uint32_t limited = 0u;
+  const bool budget = a.f.budget != nullptr;
+  const double dpg = (double)col.ps * (double)(P0 / GRAV);   // p0 ps / g; times dhs_k: dp_k / g
+  colsum[0] = 0.0;
+  colsum[1] = 0.0;
   // term-ordered sum (speedy_physics.py:123-125)
   const uint32_t tm = a.term_mask;
   const size_t ncol = a.ncol;
@@ -162,6 +169,11 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
     a.f.dv[base3 + k * ncol] = dv;
     a.f.dt[base3 + k * ncol] = t;
     a.f.dq[base3 + k * ncol] = q;
+    if (budget) {
+      const double dp = (double)a.v.dhs[k] * dpg;
+      colsum[0] += (double)CP * (double)t * dp;
+      colsum[1] += (double)q * dp;
+    }
   }
   if (DIAG) {
     float* dg = a.f.diag + (size_t)m * diag_rows(KX) * ncol + c;
@@ -355,6 +367,49 @@ __device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t bi
   }
 }
 
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Float64 budget partials of this block per member slot (layout as the
+// parameter partials, kBudParts doubles): weighted sums, the largest column
+// water residual, and the weight sum.
+__device__ __forceinline__ void block_budget_partials(const KArgs& a, uint32_t bid, uint32_t gid, bool active,
+                                                      uint32_t slot, const double v[kBudParts]) {
+  if (a.ncol < (uint32_t)kBlock) {
+    if (active)
+#pragma unroll
+      for (int i = 0; i < kBudParts; ++i) a.bud_part[(size_t)gid * kBudParts + i] = v[i];
+    return;
+  }
+  __shared__ double s_bud[kBlock / 64][2 * kBudParts];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < kBudParts; ++i) {
+      const bool mine = active && slot == (uint32_t)j;
+      const double x = i == 5 ? wave_max_d(mine ? v[i] : 0.0) : wave_sum_d(mine ? v[i] : 0.0);
+      if (lane == 0) s_bud[wave][j * kBudParts + i] = x;
+    }
+  __syncthreads();
+  if (threadIdx.x < 2 * kBudParts) {
+    const bool is_max = (threadIdx.x % kBudParts) == 5;
+    double acc = s_bud[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) acc = is_max ? fmax(acc, s_bud[w][threadIdx.x]) : acc + s_bud[w][threadIdx.x];
+    a.bud_part[(size_t)bid * 2 * kBudParts + threadIdx.x] = acc;
+  }
+}
+
 // Forward (and, with VJP, the fused forward + VJP).  FUSED: single launch with
 // the split grid barrier (all blocks co-resident); otherwise S comes from
 // levelset_kernel of the same call.
@@ -423,7 +478,16 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   load_zonal<KX>(a, c, col);
   phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
   uint32_t limited = 0u;
-  if (active) limited = store_outputs<KX, DIAG>(a, m, c, col, pa, pb);
+  double colsum[2] = {0.0, 0.0};
+  if (active) limited = store_outputs<KX, DIAG>(a, m, c, col, pa, pb, colsum);
+  if (a.f.budget) {   // float64 column budgets (block-uniform branch)
+    const uint32_t j = c - divq(c, a.dlat) * a.nlat;
+    const double w = (double)a.f.coa[j];
+    const double E = (double)pb.evap2, P = (double)pa.precnv + (double)pa.precls;
+    const double res = colsum[1] - (E - P);
+    const double v[kBudParts] = {w * colsum[0], w * colsum[1], w * E, w * P, w * res, fabs(res), w};
+    block_budget_partials(a, bid, gid, active, m - m_first, v);
+  }
   if (VJP && a.adj.g_t != nullptr) {   // batches mix forward-only and VJP segments
     ColumnCt<float, KX> ct;
     load_cotangents<KX>(a, m, c, ct);
@@ -453,6 +517,51 @@ __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
 
 // Per-member parameter gradients from the partials, one block per member, in a
 // fixed order (deterministic).
+__device__ __forceinline__ void budget_reduce_body(const KArgs& a, uint32_t m) {
+  double acc[kBudParts];
+#pragma unroll
+  for (int i = 0; i < kBudParts; ++i) acc[i] = 0.0;
+  if (a.ncol >= (uint32_t)kBlock) {
+    const uint32_t b0 = (m * a.ncol) / kBlock, b1 = (m * a.ncol + a.ncol - 1u) / kBlock;
+    for (uint32_t b = b0 + threadIdx.x; b <= b1; b += kBlock) {
+      const uint32_t j = m - divq(b * kBlock, a.dcol);
+#pragma unroll
+      for (int i = 0; i < kBudParts; ++i) {
+        const double x = a.bud_part[((size_t)b * 2 + j) * kBudParts + i];
+        acc[i] = i == 5 ? fmax(acc[i], x) : acc[i] + x;
+      }
+    }
+  } else {
+    for (uint32_t cc = threadIdx.x; cc < a.ncol; cc += kBlock)
+#pragma unroll
+      for (int i = 0; i < kBudParts; ++i) {
+        const double x = a.bud_part[((size_t)m * a.ncol + cc) * kBudParts + i];
+        acc[i] = i == 5 ? fmax(acc[i], x) : acc[i] + x;
+      }
+  }
+  __shared__ double s_red[kBlock / 64][kBudParts];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int i = 0; i < kBudParts; ++i) {
+    const double v = i == 5 ? wave_max_d(acc[i]) : wave_sum_d(acc[i]);
+    if (lane == 0) s_red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kBudParts) {
+    const bool is_max = threadIdx.x == 5;
+    double t = s_red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) t = is_max ? fmax(t, s_red[w][threadIdx.x]) : t + s_red[w][threadIdx.x];
+    s_red[0][threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < SPEEDY_NBUDGET) {
+    const double wsum = s_red[0][SPEEDY_NBUDGET];
+    const double t = s_red[0][threadIdx.x];
+    a.f.budget[(size_t)m * SPEEDY_NBUDGET + threadIdx.x] = threadIdx.x == 5 ? t : t / wsum;
+  }
+}
+
 __device__ __forceinline__ void param_reduce_body(const KArgs& a, uint32_t m) {
   float acc[VJP_NPARAM];
 #pragma unroll
@@ -485,7 +594,16 @@ __device__ __forceinline__ void param_reduce_body(const KArgs& a, uint32_t m) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) { param_reduce_body(a, blockIdx.x); }
+// Per-member sums of a call: parameter gradients (VJP) and float64 budgets.
+__device__ __forceinline__ void member_reduce_body(const KArgs& a, uint32_t m) {
+  if (a.adj.g_params) param_reduce_body(a, m);
+  if (a.f.budget) {
+    __syncthreads();
+    budget_reduce_body(a, m);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) { member_reduce_body(a, blockIdx.x); }
 
 // ---------------------------------------------------------------- batches
 // A batch descriptor holds one KArgs per segment and, per launch, a table of
@@ -508,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void fwd_batch_kernel(const KArgs* segs, co
 
 __global__ __launch_bounds__(kBlock) void param_reduce_batch_kernel(const KArgs* segs, const uint32_t* table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
-  param_reduce_body(segs[e >> kSegShift], e & kSegLocalMask);
+  member_reduce_body(segs[e >> kSegShift], e & kSegLocalMask);
 }
 
 __global__ void fill_zero_kernel(uint32_t* p, int64_t n) {
@@ -570,7 +688,7 @@ Grid make_grid(int64_t nlon, int64_t nlat, int64_t nmember) {
 }
 
 struct WsLayout {
-  size_t last_s, partial, prm, total;
+  size_t last_s, partial, prm, bud, total;
 };
 
 WsLayout ws_layout(const Grid& g, int32_t nmember) {
@@ -580,7 +698,9 @@ WsLayout ws_layout(const Grid& g, int32_t nmember) {
   const size_t npart = std::max<size_t>(2 * (size_t)g.nblk, (size_t)g.ls_blocks * g.jmax);
   L.prm = L.partial + align256(sizeof(uint32_t) * npart);
   const size_t nprm = g.ncol >= (uint32_t)kBlock ? (size_t)g.nblk * 2 * VJP_NPARAM : (size_t)g.ntot * VJP_NPARAM;
-  L.total = L.prm + align256(sizeof(float) * nprm);
+  L.bud = L.prm + align256(sizeof(float) * nprm);
+  const size_t nbud = g.ncol >= (uint32_t)kBlock ? (size_t)g.nblk * 2 * kBudParts : (size_t)g.ntot * kBudParts;
+  L.total = L.bud + align256(sizeof(double) * nbud);
   return L;
 }
 
@@ -643,6 +763,7 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   a.last_s = reinterpret_cast<uint32_t*>(w + L.last_s);
   a.partial = reinterpret_cast<uint32_t*>(w + L.partial);
   a.prm_part = reinterpret_cast<float*>(w + L.prm);
+  a.bud_part = reinterpret_cast<double*>(w + L.bud);
   return SPEEDY_OK;
 }
 
@@ -690,7 +811,7 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
     timer_end(s, slot);
     e = hipGetLastError();
   }
-  if (e == hipSuccess && VJP) {
+  if (e == hipSuccess && (VJP || a.f.budget)) {
     const int slot = timer_begin(s, 3);
     hipLaunchKernelGGL(param_reduce_kernel, dim3(a.nmember), dim3(kBlock), 0, s, a);
     timer_end(s, slot);
@@ -721,7 +842,7 @@ int batch_layout(const SpeedySegment* segs, int32_t nseg, BatchLayout& L) {
       return fail(SPEEDY_ERR_CAPACITY, "speedy_fwd_batch: segment " + std::to_string(i) + " too large for a batch");
     nb += g.nblk;
     if (sg.level_set < 0) nl += g.ls_blocks;
-    if (sg.adj.g_t) nr += (size_t)sg.nmember;
+    if (sg.adj.g_t || sg.fields.budget) nr += (size_t)sg.nmember;
   }
   if (nb > 0x7fffffffu) return fail(SPEEDY_ERR_CAPACITY, "speedy_fwd_batch: too many blocks");
   L.blk = align256(sizeof(KArgs) * (size_t)nseg);
@@ -860,7 +981,8 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
       for (uint32_t b = 0; b < a.ls_blocks; ++b) t_ls[nl++] = ((uint32_t)i << kSegShift) | b;
       jmax = std::max(jmax, a.jmax);
     }
-    if (nset) for (uint32_t m = 0; m < a.nmember; ++m) t_red[nr++] = ((uint32_t)i << kSegShift) | m;
+    if (nset || sg.fields.budget)
+      for (uint32_t m = 0; m < a.nmember; ++m) t_red[nr++] = ((uint32_t)i << kSegShift) | m;
   }
   g_last_error.clear();
   hipStream_t s = (hipStream_t)stream;

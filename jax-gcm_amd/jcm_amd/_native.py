@@ -55,7 +55,10 @@ class SpeedyDate(ctypes.Structure):
 
 
 FIELD_NAMES = ("u", "v", "t", "q", "phi", "ps", "alb0", "sice_am", "snowc_am", "soilw_am", "stl_am", "sst",
-               "fmask", "orog", "phis0", "sia", "coa", "du", "dv", "dt", "dq", "diag")
+               "fmask", "orog", "phis0", "sia", "coa", "du", "dv", "dt", "dq", "diag", "budget")
+NBUDGET = 6
+BUDGET_NAMES = ("column_energy_tendency_W_m2", "column_water_tendency_g_m2_s", "evaporation_g_m2_s",
+                "precipitation_g_m2_s", "water_residual_g_m2_s", "max_abs_column_water_residual_g_m2_s")
 
 
 class SpeedyFields(ctypes.Structure):

@@ -101,6 +101,7 @@ class SpeedyPhysics(Physics):
         self._consts: dict = {}
         self._abi_params = None
         self._abi_params_src = None
+        self.last_budgets = None
 
     # ------------------------------------------------------------------ helpers
     def _require_gpu(self):
@@ -185,14 +186,18 @@ class SpeedyPhysics(Physics):
     # ------------------------------------------------------------------ API
     def compute_tendencies(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
                            date: DateData, *, term_mask: int = _native.TERMS_ALL,
-                           level_set: int = -1, verify: bool = False) -> Tuple[PhysicsTendency, Any]:
+                           level_set: int = -1, verify: bool = False,
+                           budgets: bool = False) -> Tuple[PhysicsTendency, Any]:
         """SpeedyPhysics.compute_tendencies (speedy_physics.py:88-127).
 
         term_mask: bit i keeps term i's tendency in the sum (speedy_physics.py:67-80 order);
         level_set: >= 0 overrides the Q1 cloud-top level set (parity tests);
         verify: also apply verify_state before and verify_tendencies after, with
         dt = date.dt_seconds, as get_physical_tendencies does (physics_interface.py:371-375),
-        in the same kernel."""
+        in the same kernel;
+        budgets: also reduce the float64 column energy and water budgets (speedy_abi.h,
+        SPEEDY_NBUDGET) into ``self.last_budgets``: a float64 tensor (nmember, 6) whose
+        columns are named by ``_native.BUDGET_NAMES``."""
         self._require_gpu()
         keep: list = []
         f, t, nmember = self._fields(state, forcing, geometry, date, keep)
@@ -206,6 +211,9 @@ class SpeedyPhysics(Physics):
             f.diag = diag.data_ptr()
         else:
             f.diag = None
+        if budgets:
+            self.last_budgets = torch.empty((nmember, _native.NBUDGET), dtype=torch.float64, device=self.device)
+            f.budget = self.last_budgets.data_ptr()
         ws = self.workspace(ix, il, nmember)
         flags = (_native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0) | (VERIFY_FLAGS if verify else 0)
         rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(geometry.to_abi()), ctypes.byref(self._params()),

@@ -168,3 +168,23 @@ def run_oracle_verified(state, forcing, geometry, date, params=None, dtype=np.fl
                                           return_level_set=True)
     dt = float(np.float32(date.dt_seconds))
     return so.verify_tendencies(st, tend, dt), data, S
+
+
+def oracle_budgets(case, params=None, dtype=np.float64, verify=False):
+    """The float64 budgets of speedy_abi.h (SPEEDY_NBUDGET) computed from the oracle's
+    tendencies and data: area weight coa, dp/g = dhs ps p0 / g."""
+    st, fo, ge, da = case
+    if verify:
+        tend, data, _ = run_oracle_verified(*case, params=params, dtype=dtype)
+    else:
+        tend, data, _ = run_oracle(*case, params=params, dtype=dtype)
+    ps = np.asarray(st.normalized_surface_pressure, np.float64)
+    dp = np.asarray(ge.dhs, np.float32).astype(np.float64)[:, None, None] * ps[None] * (1e5 / 9.81)
+    ecol = (1004.0 * np.asarray(tend.temperature, np.float64) * dp).sum(0)
+    wcol = (np.asarray(tend.specific_humidity, np.float64) * dp).sum(0)
+    E = np.asarray(data.surface_flux.evap[..., 2], np.float64)
+    P = np.asarray(data.convection.precnv, np.float64) + np.asarray(data.condensation.precls, np.float64)
+    res = wcol - (E - P)
+    w = np.broadcast_to(np.asarray(ge.coa, np.float32).astype(np.float64)[None, :], ps.shape)
+    mean = lambda x: float((w * x).sum() / w.sum())
+    return np.array([mean(ecol), mean(wcol), mean(E), mean(P), mean(res), float(np.abs(res).max())])

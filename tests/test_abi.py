@@ -64,7 +64,7 @@ def _call(kx=8, nlon=4, nlat=2, nmember=1, flags=0, term_mask=_native.TERMS_ALL,
     lib = _native.load()
     f = _native.SpeedyFields()
     for i, n in enumerate(_native.FIELD_NAMES):
-        if n != "diag" and n != null_field:
+        if n not in ("diag", "budget") and n != null_field:
             setattr(f, n, 0x1000 + 64 * i)   # never dereferenced: validation fails first
     g = Geometry.from_grid_shape((64, 32), num_levels=8).to_abi()
     g.kx = kx

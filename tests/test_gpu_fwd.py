@@ -322,3 +322,26 @@ def test_fwd_fused_verify_steps(phys):
         ok, e32, e64, eref = H.parity(_np(getattr(tend, f)), getattr(ref, f), getattr(ref64, f))
         print(f"verify {f}: {e32:.2e} (fp64 {e64:.2e})", flush=True)
         assert ok, f"{f}: {e32:.3e}"
+
+
+@pytest.mark.parametrize("nlon,nlat,verify", [(256, 128, False), (64, 32, True)])
+def test_fwd_float64_budgets(phys, nlon, nlat, verify):
+    """Float64 column energy / water budgets reduced on the GPU against the same sums of
+    the oracle's tendencies: each global mean within 1e-5 of its scale, and the
+    column water budget closes (residual <= 1e-5 of the precipitation scale), as it
+    does in the float64 oracle."""
+    import torch
+    from test_host_column import _negative_q_case
+    case = _negative_q_case() if verify else synthetic.make_case(nlon, nlat, 8, seed=1, land=True)
+    phys.parameters = Parameters.default()
+    phys.compute_tendencies(*case, verify=verify, budgets=True)
+    torch.cuda.synchronize()
+    b = phys.last_budgets.cpu().numpy()[0]
+    ref = H.oracle_budgets(case, verify=verify)
+    print("budgets gpu", b, "oracle64", ref, flush=True)
+    wscale = max(abs(ref[2]), abs(ref[3]))
+    assert abs(b[0] - ref[0]) <= 1e-5 * abs(ref[0]) + 1e-3
+    for i in (1, 2, 3):
+        assert abs(b[i] - ref[i]) <= 1e-5 * wscale, i
+    assert abs(b[4]) <= 1e-5 * wscale and abs(ref[4]) <= 1e-7 * wscale
+    assert b[5] >= 0.0
