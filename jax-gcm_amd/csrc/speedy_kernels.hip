@@ -99,7 +99,7 @@ __device__ __forceinline__ uint32_t divq(uint32_t n, const Divisor& dv) {
   return q;
 }
 
-template <int KX>
+template <int KX, bool EXTRA>
 __device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t c, ColumnIn<float, KX>& col) {
   const size_t ncol = a.ncol;
   const size_t base3 = (size_t)m * KX * ncol + c;
@@ -109,7 +109,7 @@ __device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t 
     col.Q[k] = a.f.q[base3 + k * ncol];
     col.PHI[k] = a.f.phi[base3 + k * ncol];
   }
-  if (a.flags & SPEEDY_FLAG_VERIFY_STATE) {
+  if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_STATE)) {
 #pragma unroll
     for (int k = 0; k < KX; ++k) col.Q[k] = verify_q(col.Q[k]);
   }
@@ -136,12 +136,12 @@ __device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<
 }
 
 // Returns the levels whose q tendency verify_tendencies limited (bit k).
-template <int KX, bool DIAG>
+template <int KX, bool DIAG, bool EXTRA>
 __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, uint32_t c,
                                                   const ColumnIn<float, KX>& col, const PhaseA<float, KX>& pa,
                                                   const PhaseB<float, KX>& pb, double colsum[2]) {
   uint32_t limited = 0u;
-  const bool budget = a.f.budget != nullptr;
+  const bool budget = EXTRA && a.f.budget != nullptr;
   const double dpg = (double)col.ps * (double)(P0 / GRAV);   // p0 ps / g; times dhs_k: dp_k / g
   colsum[0] = 0.0;
   colsum[1] = 0.0;
@@ -160,7 +160,7 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
     }
     if (tm & (1u << SPEEDY_TERM_LW_UP)) t = t + pb.tlw[k];
     if (tm & (1u << SPEEDY_TERM_VDIFF)) { t = t + pb.tvd[k]; q = q + pb.qvd[k]; }
-    if (a.flags & SPEEDY_FLAG_VERIFY_TENDENCIES) {
+    if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_TENDENCIES)) {
       bool lim;
       q = verify_dq(a.f.q[base3 + k * ncol], q, a.u.dt_step, lim);   // the unclamped q (re-read, L2)
       limited |= (lim ? 1u : 0u) << k;
@@ -413,7 +413,9 @@ __device__ __forceinline__ void block_budget_partials(const KArgs& a, uint32_t b
 // Forward (and, with VJP, the fused forward + VJP).  FUSED: single launch with
 // the split grid barrier (all blocks co-resident); otherwise S comes from
 // levelset_kernel of the same call.
-template <int KX, bool DIAG, bool FUSED, bool VJP>
+// EXTRA: the optional verify steps and float64 budgets are compiled in (their
+// flags and pointers are then read at run time); the plain forward omits them.
+template <int KX, bool DIAG, bool FUSED, bool VJP, bool EXTRA>
 __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   __shared__ uint32_t s_or[2], s_s[2], s_tag;
   const uint32_t gid = bid * kBlock + threadIdx.x;
@@ -425,7 +427,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   const uint32_t m_first = divq(blk0, a.dcol);
   const uint32_t nm = divq(min(blk0 + kBlock, a.ntot) - 1u, a.dcol) - m_first + 1u;   // <= 2 when fused
   ColumnIn<float, KX> col;
-  load_state<KX>(a, m, c, col);
+  load_state<KX, EXTRA>(a, m, c, col);
   PhaseA<float, KX> pa;
   phase_a<float, KX>(col, a.p, a.v, a.u, pa);
   const bool need_s = a.level_set < 0;
@@ -479,8 +481,8 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
   uint32_t limited = 0u;
   double colsum[2] = {0.0, 0.0};
-  if (active) limited = store_outputs<KX, DIAG>(a, m, c, col, pa, pb, colsum);
-  if (a.f.budget) {   // float64 column budgets (block-uniform branch)
+  if (active) limited = store_outputs<KX, DIAG, EXTRA>(a, m, c, col, pa, pb, colsum);
+  if (EXTRA && a.f.budget) {   // float64 column budgets (block-uniform branch)
     const uint32_t j = c - divq(c, a.dlat) * a.nlat;
     const double w = (double)a.f.coa[j];
     const double E = (double)pb.evap2, P = (double)pa.precnv + (double)pa.precls;
@@ -492,13 +494,13 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
     ColumnCt<float, KX> ct;
     load_cotangents<KX>(a, m, c, ct);
     float g_qraw[KX];
-    verify_ct<float, KX>(a.flags, limited, a.u.dt_step, ct, g_qraw);
+    verify_ct<float, KX>(EXTRA ? a.flags : 0u, limited, a.u.dt_step, ct, g_qraw);
     ColumnAdj<float, KX> g;
     // longwave flux tapes in LDS, [slot][thread]: the kernel runs one block per CU
     __shared__ float s_tape[vjp_tape_floats<KX>() * kBlock];
     LdsStore tape{s_tape + threadIdx.x};
     column_vjp<float, KX>(col, pa, pb, S, a.p, a.v, a.u, a.flags, ct, g, tape);
-    if (a.flags & SPEEDY_FLAG_VERIFY_STATE) {
+    if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_STATE)) {
       const size_t base3 = (size_t)m * KX * a.ncol + c;
 #pragma unroll
       for (int k = 0; k < KX; ++k) g.Q[k] = verify_q_grad(a.f.q[base3 + k * (size_t)a.ncol], g.Q[k]);
@@ -510,9 +512,9 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   }
 }
 
-template <int KX, bool DIAG, bool FUSED, bool VJP>
+template <int KX, bool DIAG, bool FUSED, bool VJP, bool EXTRA>
 __global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
-  fwd_body<KX, DIAG, FUSED, VJP>(a, blockIdx.x);
+  fwd_body<KX, DIAG, FUSED, VJP, EXTRA>(a, blockIdx.x);
 }
 
 // Per-member parameter gradients from the partials, one block per member, in a
@@ -621,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void levelset_batch_kernel(const KArgs* seg
 template <int KX>
 __global__ __launch_bounds__(kBlock) void fwd_batch_kernel(const KArgs* segs, const uint32_t* table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
-  fwd_body<KX, false, false, true>(segs[e >> kSegShift], e & kSegLocalMask);
+  fwd_body<KX, false, false, true, true>(segs[e >> kSegShift], e & kSegLocalMask);
 }
 
 __global__ __launch_bounds__(kBlock) void param_reduce_batch_kernel(const KArgs* segs, const uint32_t* table) {
@@ -767,6 +769,22 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   return SPEEDY_OK;
 }
 
+// Make sure the device's per-thread stack limit covers the kernel's private
+// segment (register spills of the adjoint kernels: up to ~4 KB per lane at 16
+// levels; HIP's default limit is 1 KB).  Called once per kernel before its
+// first launch; raises the limit only.
+template <typename K>
+hipError_t ensure_stack(K kernel) {
+  hipFuncAttributes fa;
+  hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kernel));
+  if (e != hipSuccess) return e;
+  size_t cur = 0;
+  e = hipDeviceGetLimit(&cur, hipLimitStackSize);
+  if (e != hipSuccess) return e;
+  if ((size_t)fa.localSizeBytes > cur) e = hipDeviceSetLimit(hipLimitStackSize, (size_t)fa.localSizeBytes);
+  return e;
+}
+
 // Largest grid that is co-resident on the current device for a kernel.
 template <typename K>
 int64_t coresident_blocks(K kernel) {
@@ -777,10 +795,17 @@ int64_t coresident_blocks(K kernel) {
   return (int64_t)per_cu * ncu;
 }
 
-template <int KX, bool DIAG, bool VJP>
+template <int KX, bool DIAG, bool VJP, bool EXTRA>
 int launch_fwd(const KArgs& a, hipStream_t s) {
   static int64_t cap = -1;   // one GPU per process in this library's deployments
-  if (cap < 0) cap = coresident_blocks(fwd_kernel<KX, DIAG, true, VJP>);
+  if (cap < 0) {
+    if (VJP) {
+      hipError_t se = ensure_stack(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
+      if (se == hipSuccess) se = ensure_stack(fwd_kernel<KX, DIAG, false, VJP, EXTRA>);
+      if (se != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: stack limit: ") + hipGetErrorString(se));
+    }
+    cap = coresident_blocks(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
+  }
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(s, &cs);
   const bool fused = a.level_set < 0 && (int64_t)a.nblk <= cap && a.ncol >= (uint32_t)kBlock;
@@ -790,12 +815,13 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
     const int slot = timer_begin(s, kind);
     if (cs == hipStreamCaptureStatusNone) {
       void* args[] = {const_cast<KArgs*>(&a)};
-      e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&fwd_kernel<KX, DIAG, true, VJP>), dim3(a.nblk),
+      e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&fwd_kernel<KX, DIAG, true, VJP, EXTRA>),
+                                     dim3(a.nblk),
                                      dim3(kBlock), args, 0, s);
     } else {
       // Graph capture: a plain launch of a grid that fits on the device; the
       // level-set poll is bounded, so the kernel terminates in every case.
-      hipLaunchKernelGGL((fwd_kernel<KX, DIAG, true, VJP>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL((fwd_kernel<KX, DIAG, true, VJP, EXTRA>), dim3(a.nblk), dim3(kBlock), 0, s, a);
       e = hipGetLastError();
     }
     timer_end(s, slot);
@@ -807,7 +833,7 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
       timer_end(s, slot);
     }
     slot = timer_begin(s, kind);
-    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false, VJP>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false, VJP, EXTRA>), dim3(a.nblk), dim3(kBlock), 0, s, a);
     timer_end(s, slot);
     e = hipGetLastError();
   }
@@ -821,9 +847,20 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
   return SPEEDY_OK;
 }
 
+bool needs_extra(const KArgs& a) {
+  return (a.flags & (SPEEDY_FLAG_VERIFY_STATE | SPEEDY_FLAG_VERIFY_TENDENCIES)) || a.f.budget != nullptr;
+}
+
 template <int KX>
 int launch_fwd_kx(const KArgs& a, hipStream_t s) {
-  return a.f.diag ? launch_fwd<KX, true, false>(a, s) : launch_fwd<KX, false, false>(a, s);
+  if (needs_extra(a))
+    return a.f.diag ? launch_fwd<KX, true, false, true>(a, s) : launch_fwd<KX, false, false, true>(a, s);
+  return a.f.diag ? launch_fwd<KX, true, false, false>(a, s) : launch_fwd<KX, false, false, false>(a, s);
+}
+
+template <int KX>
+int launch_fwd_vjp_kx(const KArgs& a, hipStream_t s) {
+  return needs_extra(a) ? launch_fwd<KX, false, true, true>(a, s) : launch_fwd<KX, false, true, false>(a, s);
 }
 
 struct BatchLayout {
@@ -938,9 +975,9 @@ int speedy_fwd_vjp(const SpeedyFields* fields, const SpeedyAdjoint* adj, const S
   g_last_error.clear();
   hipStream_t s = (hipStream_t)stream;
   switch (vert->kx) {
-    case 7: return launch_fwd<7, false, true>(a, s);
-    case 8: return launch_fwd<8, false, true>(a, s);
-    default: return launch_fwd<16, false, true>(a, s);
+    case 7: return launch_fwd_vjp_kx<7>(a, s);
+    case 8: return launch_fwd_vjp_kx<8>(a, s);
+    default: return launch_fwd_vjp_kx<16>(a, s);
   }
 }
 
@@ -994,6 +1031,8 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
   const uint32_t* d_ls = reinterpret_cast<const uint32_t*>(d + L.ls);
   const uint32_t* d_red = reinterpret_cast<const uint32_t*>(d + L.red);
   auto launch = [&](auto ls_kernel, auto fwd_kernel_b) {
+    static bool stack_ok = false;
+    if (!stack_ok && ensure_stack(fwd_kernel_b) == hipSuccess) stack_ok = true;
     if (nl) {
       const int slot = timer_begin(s, 0);
       hipLaunchKernelGGL(ls_kernel, dim3(nl), dim3(kBlock), jmax * sizeof(uint32_t), s, d_args, d_ls);
