@@ -124,6 +124,9 @@ class PhysicsServer:
         self.n_batches = 0
         self.batch_sizes: list = []
         self._ws_free: dict = {}      # (nlon, nlat, nmember) -> [workspace tensors]
+        self._ws_bytes: dict = {}
+        self._members: dict = {}
+        self._verts: dict = {}
 
     # ------------------------------------------------------------ resources
     def _workspace(self, key, stream) -> torch.Tensor:
@@ -139,9 +142,65 @@ class PhysicsServer:
         return ws
 
     def footprint(self, req: Request) -> int:
-        kx, ix, il = req.geometry.nodal_shape
-        nmember = req.columns // (ix * il)
-        return request_footprint(req, kx, self.lib.speedy_workspace_bytes(ix, il, nmember))
+        fp = getattr(req, "_fp", None)
+        if fp is None:
+            kx, ix, il = req.geometry.nodal_shape
+            key = (ix, il, req.columns // (ix * il))
+            wsb = self._ws_bytes.get(key)
+            if wsb is None:
+                wsb = self._ws_bytes[key] = self.lib.speedy_workspace_bytes(*key)
+            fp = req._fp = request_footprint(req, kx, wsb)
+        return fp
+
+    def _member(self, req: Request):
+        """Input half of the call structs for a member's (state, forcing, geometry, day),
+        built once: members' states stay resident between requests."""
+        key = (id(req.state), id(req.forcing), id(req.geometry), req.date.model_day())
+        hit = self._members.get(key)
+        if hit is None:
+            keep: list = []
+            f, t, nm = self.phys._fields(req.state, req.forcing, req.geometry, req.date, keep)
+            kx, ix, il = req.geometry.nodal_shape
+            flags = _native.FLAG_LFLUXLAND if bool(req.forcing.lfluxland) else 0
+            hit = (f, tuple(t.shape), (ix, il, nm), flags, keep, (req.state, req.forcing, req.geometry))
+            if len(self._members) > 4096:
+                self._members.clear()
+            self._members[key] = hit
+        return hit
+
+    def _prepare(self, req: Request, seg):
+        """Fill one batch segment for req; allocates its outputs in one block."""
+        f_in, tshape, (ix, il, nm), flags, _, _ = self._member(req)
+        f = _native.SpeedyFields()
+        ctypes.pointer(f)[0] = f_in
+        n3 = int(np.prod(tshape))
+        lead = tshape[:-3]
+        vjp = req.kind == KIND_VJP
+        n2 = nm * ix * il
+        total = (8 * n3 + n2 + nm * 5) if vjp else 4 * n3
+        buf = torch.empty(total, dtype=torch.float32, device=self.dev)
+        views = [buf[i * n3:(i + 1) * n3].view(tshape) for i in range(8 if vjp else 4)]
+        f.du, f.dv, f.dt, f.dq = (x.data_ptr() for x in views[:4])
+        req.tendency = PhysicsTendency(*views[:4])
+        seg.fields = f
+        if vjp:
+            adj = _native.SpeedyAdjoint()
+            ct = req.cotangent
+            cts = [ct.u_wind, ct.v_wind, ct.temperature, ct.specific_humidity]
+            for c in cts:
+                if not (isinstance(c, torch.Tensor) and c.device == self.dev and c.dtype == torch.float32
+                        and c.is_contiguous() and tuple(c.shape) == tshape):
+                    raise ValueError("cotangents must be contiguous float32 device tensors of the state's shape")
+            adj.ct_du, adj.ct_dv, adj.ct_dt, adj.ct_dq = (c.data_ptr() for c in cts)
+            g_ps = buf[8 * n3:8 * n3 + n2].view(lead + (ix, il))
+            g_par = buf[8 * n3 + n2:].view(lead + (5,))
+            adj.g_u, adj.g_v, adj.g_t, adj.g_q = (x.data_ptr() for x in views[4:8])
+            adj.g_ps, adj.g_params = g_ps.data_ptr(), g_par.data_ptr()
+            seg.adj = adj
+            req.state_grads = PhysicsState(views[4], views[5], views[6], views[7], None, g_ps)
+            from .params import VJP_PARAMS
+            req.param_grads = {f"{g}.{n}": g_par[..., i] for i, (g, n) in enumerate(VJP_PARAMS)}
+        return (ix, il, nm), flags, buf
 
     # ------------------------------------------------------------ loop
     def submit(self, req: Request):
@@ -182,23 +241,19 @@ class PhysicsServer:
         slot.stream.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(slot.stream):
             for i, req in enumerate(batch):
-                if req.kind == KIND_VJP:
-                    f, adj, outs, (ix, il, nm), flags = phys.prepare_vjp(req.state, req.forcing, req.geometry,
-                                                                         req.date, req.cotangent, keep)
-                    segs[i].adj = adj
-                    req.state_grads, req.param_grads = outs["state_grads"], outs["param_grads"]
-                else:
-                    f, outs, (ix, il, nm), flags = phys.prepare_fwd(req.state, req.forcing, req.geometry,
-                                                                    req.date, keep)
-                req.tendency = outs["tendency"]
+                (ix, il, nm), flags, buf = self._prepare(req, segs[i])
+                keep.append(buf)
                 ws = self._workspace((ix, il, nm), slot.stream)
                 req._ws = ((ix, il, nm), ws)
-                segs[i].fields = f
                 segs[i].date = req.date.to_abi()
                 segs[i].nlon, segs[i].nlat, segs[i].nmember = ix, il, nm
                 segs[i].flags, segs[i].level_set = flags, -1
                 segs[i].workspace, segs[i].workspace_bytes = ws.data_ptr(), ws.numel()
-            vert = batch[0].geometry.to_abi()
+            gkey = id(batch[0].geometry)
+            hit = self._verts.get(gkey)
+            if hit is None or hit[0] is not batch[0].geometry:
+                hit = self._verts[gkey] = (batch[0].geometry, batch[0].geometry.to_abi())
+            vert = hit[1]
             for req in batch[1:]:
                 if req.geometry.nodal_shape[0] != batch[0].geometry.nodal_shape[0]:
                     raise _native.SpeedyError("a batch shares one vertical grid")

@@ -164,9 +164,12 @@ def test_vjp_finite_at_unit_cotangents(phys):
     assert np.all(np.isfinite(prm))
 
 
-@pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3), pytest.param(256, 128, 8, marks=pytest.mark.skip(
-    reason="same T85 x 8-member two-launch shape as the forward case kept off the GPU (see test_gpu_fwd.T85X8_SKIP)")),
-    (8, 4, 6)])
+_TWO_LAUNCH = "same shapes and non-batched two-launch path as the forward cases kept off the GPU (test_gpu_fwd)"
+
+
+@pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3),
+                                               pytest.param(256, 128, 8, marks=pytest.mark.skip(reason=_TWO_LAUNCH)),
+                                               pytest.param(8, 4, 6, marks=pytest.mark.skip(reason=_TWO_LAUNCH))])
 def test_vjp_members_stacked(phys, nlon, nlat, nmember):
     """Stacked members (fused path; beyond one co-resident launch; narrower than a
     workgroup): each member's gradients equal its own float64 adjoint, with its
