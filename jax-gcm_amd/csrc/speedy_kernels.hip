@@ -801,7 +801,7 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
   if (cap < 0) {
     if (VJP) {
       hipError_t se = ensure_stack(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
-      if (se == hipSuccess) se = ensure_stack(fwd_kernel<KX, DIAG, false, VJP, EXTRA>);
+      if (se == hipSuccess) se = ensure_stack(fwd_kernel<KX, DIAG, false, VJP, true>);
       if (se != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: stack limit: ") + hipGetErrorString(se));
     }
     cap = coresident_blocks(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
@@ -832,8 +832,10 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
       hipLaunchKernelGGL(levelset_kernel<KX>, dim3(a.ls_blocks), dim3(kBlock), a.jmax * sizeof(uint32_t), s, a);
       timer_end(s, slot);
     }
+    // The two-launch forward always carries the optional verify / budget code
+    // (run-time flags), whatever EXTRA is: one kernel per (KX, DIAG, VJP).
     slot = timer_begin(s, kind);
-    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false, VJP, EXTRA>), dim3(a.nblk), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false, VJP, true>), dim3(a.nblk), dim3(kBlock), 0, s, a);
     timer_end(s, slot);
     e = hipGetLastError();
   }

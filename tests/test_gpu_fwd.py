@@ -277,9 +277,9 @@ def test_fwd_graph_replay_matches_eager(nmember):
 
 
 TWO_LAUNCH_SKIP = ("GPU memory faults were reported at the first synchronize of this test twice (T85 x 8 members, "
-                   "then 8 x 4 x 6 members), each time right after the graph-replay test, on builds where the same "
-                   "sequence had passed before with a superset of these kernels' memory operations; a CPU simulation "
-                   "of every index of the path is in bounds.  No cause has been shown, so the test stays off the GPU.")
+                   "then 8 x 4 x 6 members), each time right after the graph-replay test, with a plain-forward "
+                   "two-launch kernel that has since been taken out; a CPU simulation of every index of the path is "
+                   "in bounds.  No cause has been shown, so the test stays off the GPU.")
 
 
 @pytest.mark.skip(reason=TWO_LAUNCH_SKIP)
