@@ -1032,9 +1032,11 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
   const uint32_t* d_blk = reinterpret_cast<const uint32_t*>(d + L.blk);
   const uint32_t* d_ls = reinterpret_cast<const uint32_t*>(d + L.ls);
   const uint32_t* d_red = reinterpret_cast<const uint32_t*>(d + L.red);
-  auto launch = [&](auto ls_kernel, auto fwd_kernel_b) {
-    static bool stack_ok = false;
-    if (!stack_ok && ensure_stack(fwd_kernel_b) == hipSuccess) stack_ok = true;
+  // One flag per level count: the three forward kernels share this lambda's
+  // instantiation (same pointer types) but not their private segment sizes.
+  static bool stack_ok[3] = {false, false, false};
+  auto launch = [&](auto ls_kernel, auto fwd_kernel_b, int kxi) {
+    if (!stack_ok[kxi] && ensure_stack(fwd_kernel_b) == hipSuccess) stack_ok[kxi] = true;
     if (nl) {
       const int slot = timer_begin(s, 0);
       hipLaunchKernelGGL(ls_kernel, dim3(nl), dim3(kBlock), jmax * sizeof(uint32_t), s, d_args, d_ls);
@@ -1050,9 +1052,9 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
     }
   };
   switch (vert->kx) {
-    case 7: launch(levelset_batch_kernel<7>, fwd_batch_kernel<7>); break;
-    case 8: launch(levelset_batch_kernel<8>, fwd_batch_kernel<8>); break;
-    default: launch(levelset_batch_kernel<16>, fwd_batch_kernel<16>); break;
+    case 7: launch(levelset_batch_kernel<7>, fwd_batch_kernel<7>, 0); break;
+    case 8: launch(levelset_batch_kernel<8>, fwd_batch_kernel<8>, 1); break;
+    default: launch(levelset_batch_kernel<16>, fwd_batch_kernel<16>, 2); break;
   }
   e = hipGetLastError();
   if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_batch: launch failed: ") + hipGetErrorString(e));
