@@ -57,7 +57,7 @@ class Request:
     tendency: Optional[PhysicsTendency] = None
     state_grads: Optional[PhysicsState] = None
     param_grads: Optional[dict] = None
-    level_set: Optional[int] = None
+    level_set: Optional[list] = None   # per member, when the server records level sets
     t_admit: Optional[float] = None
     t_done: Optional[float] = None
     batch: int = -1
@@ -107,7 +107,8 @@ class PhysicsServer:
 
     def __init__(self, physics: SpeedyPhysics | None = None, hbm_budget: int = 8 << 30,
                  max_batch_columns: int = 1 << 20, max_segments: int = 64, n_streams: int = 2,
-                 batching: bool = True, clock: Callable[[], float] = time.perf_counter):
+                 batching: bool = True, clock: Callable[[], float] = time.perf_counter,
+                 record_level_sets: bool = False):
         self.phys = physics or SpeedyPhysics()
         self.dev = self.phys.device
         self.lib = self.phys._lib
@@ -116,6 +117,7 @@ class PhysicsServer:
         self.max_segments = min(int(max_segments), _native.BATCH_MAX_SEGMENTS)
         self.batching = batching
         self.clock = clock
+        self.record_level_sets = record_level_sets   # read each request's Q1 level sets at completion
         self.slots = [_Slot(stream=torch.cuda.Stream(device=self.dev)) for _ in range(max(1, n_streams))]
         self.queue: deque = deque()
         self.in_flight_bytes = 0
@@ -305,12 +307,21 @@ class PhysicsServer:
                     self.in_flight_bytes -= self.footprint(req)
                     if req._ws is not None:
                         key, ws = req._ws
+                        if self.record_level_sets:
+                            req.level_set = self._read_level_sets(ws, key[2], slot.stream)
                         self._ws_free.setdefault(key, []).append(ws)
                     req._ws = None
                     self.done.append(req)
                     n += 1
                 slot.batch, slot.keep, slot.busy = [], [], False
         return n
+
+    def _read_level_sets(self, ws: torch.Tensor, nmember: int, stream) -> list:
+        buf = (ctypes.c_uint32 * nmember)()
+        _native.check(self.lib.speedy_read_level_sets(ws.data_ptr(), ws.numel(), nmember, buf,
+                                                      ctypes.c_void_p(stream.cuda_stream)),
+                      "speedy_read_level_sets")
+        return list(buf)
 
     def step(self) -> bool:
         """Launch one batch if a stream is free and a request can be admitted."""

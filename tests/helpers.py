@@ -52,6 +52,40 @@ def run_oracle(state, forcing, geometry, date, params=None, dtype=np.float32, le
     return so.compute_tendencies(st, fo, g, d, p, level_set=level_set, term_mask=term_mask, return_level_set=True)
 
 
+STATE_FIELDS = ("u_wind", "v_wind", "temperature", "specific_humidity", "geopotential",
+                "normalized_surface_pressure")
+
+
+def subgrid(case, I, J, member=None):
+    """The columns I x J (longitude, latitude indices) of a case as a case of its
+    own, of one member when `member` is given.  Every term is column-local except
+    Q1's level set, and the latitude tables follow the chosen latitudes, so with
+    the full grid's level set injected the sub-case reproduces the full grid at
+    those columns (test_oracle_subgrid pins this)."""
+    from jcm_amd.geometry import Geometry
+    st, fo, ge, da = case
+    I, J = np.asarray(I), np.asarray(J)
+    multi = np.asarray(st.temperature).ndim == 4
+
+    def cut(a, state=False):
+        a = np.asarray(a)
+        if state and multi:
+            a = a[member] if member is not None else a
+        return np.ascontiguousarray(a[..., I, :][..., J])
+
+    def cutf(a):   # (ix, il) or a climatology (ix, il, ndays)
+        a = np.asarray(a)
+        return np.ascontiguousarray(a[I][:, J])
+
+    st2 = st.copy(**{k: cut(getattr(st, k), True) for k in STATE_FIELDS})
+    fo2 = fo.copy(**{k: cutf(getattr(fo, k)) for k in ("alb0", "sice_am", "snowc_am", "soilw_am", "stl_am",
+                                                        "sea_surface_temperature")})
+    ge2 = Geometry.from_latitudes(np.asarray(ge.radang)[J], len(I), ge.nodal_shape[0], orography=cutf(ge.orog),
+                                  fmask=cutf(ge.fmask), phis0=cutf(ge.phis0), sia=np.asarray(ge.sia)[J],
+                                  coa=np.asarray(ge.coa)[J])
+    return st2, fo2, ge2, da
+
+
 def rel_inf(a, ref):
     a = np.asarray(a, np.float64)
     ref = np.asarray(ref, np.float64)
