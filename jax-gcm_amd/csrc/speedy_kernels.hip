@@ -135,8 +135,22 @@ __device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<
   col.fsol = z[0]; col.ozupp = z[1]; col.ozone = z[2]; col.zenit = z[3]; col.stratz = z[4]; col.sqcoa = z[5];
 }
 
+// u and v tendencies come from the surface stress at the lowest level only
+// (surface_flux.py:294-295); the zeros above it are stored at the start of the
+// kernel, off the critical path.
+template <int KX>
+__device__ __forceinline__ void store_uv_zeros(const KArgs& a, uint32_t m, uint32_t c) {
+  const size_t ncol = a.ncol;
+  const size_t base3 = (size_t)m * KX * ncol + c;
+#pragma unroll
+  for (int k = 0; k < KX - 1; ++k) {
+    a.f.du[base3 + k * ncol] = 0.0f;
+    a.f.dv[base3 + k * ncol] = 0.0f;
+  }
+}
+
 // Returns the levels whose q tendency verify_tendencies limited (bit k).
-template <int KX, bool DIAG, bool EXTRA>
+template <int KX, bool DIAG, bool EXTRA, bool UV_ALL>
 __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, uint32_t c,
                                                   const ColumnIn<float, KX>& col, const PhaseA<float, KX>& pa,
                                                   const PhaseB<float, KX>& pb, double colsum[2]) {
@@ -165,8 +179,10 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
       q = verify_dq(a.f.q[base3 + k * ncol], q, a.u.dt_step, lim);   // the unclamped q (re-read, L2)
       limited |= (lim ? 1u : 0u) << k;
     }
-    a.f.du[base3 + k * ncol] = du;
-    a.f.dv[base3 + k * ncol] = dv;
+    if (UV_ALL || k == KX - 1) {   // without UV_ALL the levels above were zeroed by store_uv_zeros
+      a.f.du[base3 + k * ncol] = du;
+      a.f.dv[base3 + k * ncol] = dv;
+    }
     a.f.dt[base3 + k * ncol] = t;
     a.f.dq[base3 + k * ncol] = q;
     if (budget) {
@@ -417,7 +433,7 @@ __device__ __forceinline__ void block_budget_partials(const KArgs& a, uint32_t b
 // flags and pointers are then read at run time); the plain forward omits them.
 template <int KX, bool DIAG, bool FUSED, bool VJP, bool EXTRA>
 __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
-  __shared__ uint32_t s_or[2], s_s[2], s_tag;
+  __shared__ uint32_t s_s[2];
   const uint32_t gid = bid * kBlock + threadIdx.x;
   const bool active = gid < a.ntot;
   const uint32_t g0 = gid < a.ntot ? gid : a.ntot - 1u;   // inactive lanes shadow a valid column
@@ -426,24 +442,36 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   const uint32_t blk0 = bid * kBlock;
   const uint32_t m_first = divq(blk0, a.dcol);
   const uint32_t nm = divq(min(blk0 + kBlock, a.ntot) - 1u, a.dcol) - m_first + 1u;   // <= 2 when fused
+  const bool need_s = a.level_set < 0;
+  // The call's generation tag, read first so that its latency overlaps the state
+  // loads; wave 0 publishes and gathers, so each of its lanes holds the tag.
+  uint32_t tag = 0u;
+  if (FUSED && need_s && threadIdx.x < 64u)
+    tag = gen_tag(__hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   ColumnIn<float, KX> col;
   load_state<KX, EXTRA>(a, m, c, col);
+  if (!VJP) {
+    load_zonal<KX>(a, c, col);   // S-independent; its loads travel with the state's
+    if (active) store_uv_zeros<KX>(a, m, c);
+  }
   PhaseA<float, KX> pa;
   phase_a<float, KX>(col, a.p, a.v, a.u, pa);
-  const bool need_s = a.level_set < 0;
   if (FUSED && need_s) {
-    // publish this block's word(s) for the call's level set
+    // publish this block's word(s) for the call's level set: a wave-level OR per
+    // member slot (a fused block spans at most two members), then one word per slot
+    __shared__ uint32_t s_wor[kBlock / 64][2];
+    const uint32_t bit = active ? clip_bit(pa.icltop, KX) : 0u;
+    const uint32_t o0 = wave_or(m == m_first ? bit : 0u), o1 = wave_or(m != m_first ? bit : 0u);
+    if ((threadIdx.x & 63u) == 0) { s_wor[threadIdx.x >> 6][0] = o0; s_wor[threadIdx.x >> 6][1] = o1; }
+    __syncthreads();
     if (threadIdx.x == 0) {
-      s_or[0] = 0u; s_or[1] = 0u;
-      s_tag = gen_tag(__hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      for (uint32_t j = 0; j < nm; ++j) {
+        uint32_t w = 0u;
+#pragma unroll
+        for (int i = 0; i < kBlock / 64; ++i) w |= s_wor[i][j];
+        __hip_atomic_store(&a.partial[2u * bid + j], (tag << 16) | w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
-    __syncthreads();
-    if (active) atomicOr(&s_or[m - m_first], clip_bit(pa.icltop, KX));
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (uint32_t j = 0; j < nm; ++j)
-        __hip_atomic_store(&a.partial[2u * bid + j], (s_tag << 16) | s_or[j], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
   }
   PhaseB<float, KX> pb;
   phase_b1<float, KX>(col, pa, a.p, a.v, a.u, pb);
@@ -452,14 +480,14 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   if (need_s) {
     if (threadIdx.x < 64u) {   // wave 0 gathers S for the block's member(s)
       for (uint32_t j = 0; wide && j < nm; ++j) {
-        const uint32_t v = FUSED ? collect_fused(a, s_tag, m_first + j) : collect_chunks(a, m_first + j);
+        const uint32_t v = FUSED ? collect_fused(a, tag, m_first + j) : collect_chunks(a, m_first + j);
         if (threadIdx.x == 0) s_s[j] = v;
       }
       if (bid == 0) {
         // every member's S for speedy_read_level_sets; in the fused path this also
         // waits for every block's word, i.e. every block has read the generation
         for (uint32_t mm = 0; mm < a.nmember; ++mm) {
-          const uint32_t v = FUSED ? collect_fused(a, s_tag, mm) : collect_chunks(a, mm);
+          const uint32_t v = FUSED ? collect_fused(a, tag, mm) : collect_chunks(a, mm);
           if (threadIdx.x == 0) a.last_s[mm] = v;
         }
         if (FUSED && threadIdx.x == 0) {
@@ -477,11 +505,14 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
       for (uint32_t b = b0; b <= b1; ++b) S |= a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
     }
   }
-  load_zonal<KX>(a, c, col);
+  // The forward + VJP kernels keep the zonal loads and all output stores here:
+  // moving them earlier changed their register allocation and, at 16 levels,
+  // gave wrong gradients on gfx950 (the kernels spill ~4 KB per lane).
+  if (VJP) load_zonal<KX>(a, c, col);
   phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
   uint32_t limited = 0u;
   double colsum[2] = {0.0, 0.0};
-  if (active) limited = store_outputs<KX, DIAG, EXTRA>(a, m, c, col, pa, pb, colsum);
+  if (active) limited = store_outputs<KX, DIAG, EXTRA, VJP>(a, m, c, col, pa, pb, colsum);
   if (EXTRA && a.f.budget) {   // float64 column budgets (block-uniform branch)
     const uint32_t j = c - divq(c, a.dlat) * a.nlat;
     const double w = (double)a.f.coa[j];

@@ -83,6 +83,10 @@ def test_vjp_matches_float64_adjoint(phys, name):
     kx, ix, il = case[2].nodal_shape
     cot = _cot((kx, ix, il), 5)
     tend, g, prm = _gpu_vjp(phys, case, params, cot)
+    tend2, g2, prm2 = _gpu_vjp(phys, case, params, cot)   # a second call: bit-identical (no races)
+    for k in g:
+        assert np.array_equal(g[k], g2[k]), k
+    assert np.array_equal(prm, prm2)
     S = phys.level_sets(ix, il, 1)[0]
     _, _, S_ref = H.run_oracle(*case, params=params)
     assert S == S_ref
