@@ -162,7 +162,8 @@ def serving_runs(phys, dev, n_requests, rates, seed=7):
         cot = PhysicsTendency(*[torch.randn(st.temperature.shape, device=dev) for _ in range(4)])
         pool.append((st, fo, ge, da, cot))
     out = {"workload": f"{n_requests} requests, resolution uniform over T21/T31/T42/T85 x 8 levels, "
-                       "50/50 forward and forward+VJP, Poisson arrivals, 2 streams, 4 GiB HBM budget",
+                       "50/50 forward and forward+VJP, Poisson arrivals, 2 streams, 4 GiB HBM budget; "
+                       "measured after one untimed pass of the same stream",
            "runs": []}
     for rate in rates:
         rng = np.random.default_rng(seed)
@@ -171,17 +172,69 @@ def serving_runs(phys, dev, n_requests, rates, seed=7):
         kinds = rng.integers(0, 2, n_requests)
         run = {"rate_per_s": rate}
         for mode, batching in (("batched", True), ("per_request", False)):
-            reqs = []
-            for i in range(n_requests):
-                st, fo, ge, da, cot = pool[picks[i]]
-                reqs.append(Request(state=st, forcing=fo, geometry=ge, date=da,
-                                    cotangent=cot if kinds[i] else None, arrival=float(arrivals[i]), rid=i))
+            def stream():
+                reqs = []
+                for i in range(n_requests):
+                    st, fo, ge, da, cot = pool[picks[i]]
+                    reqs.append(Request(state=st, forcing=fo, geometry=ge, date=da,
+                                        cotangent=cot if kinds[i] else None, arrival=float(arrivals[i]), rid=i))
+                return reqs
             server = PhysicsServer(phys, hbm_budget=4 << 30, n_streams=2, batching=batching)
+            # warm-up: the same stream once, untimed, so that workspaces, batch
+            # descriptors and member structs exist before the measured run
+            server.run(stream(), realtime=True, timeout=60.0)
+            torch.cuda.synchronize()
+            server.reset_stats()
+            reqs = stream()
             server.run(reqs, realtime=True, timeout=60.0)
             torch.cuda.synchronize()
             run[mode] = server.stats(reqs)
         out["runs"].append(run)
     return out
+
+
+def profile_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/<name>_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", name + "_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
+def large_leg(phys, dev, reps=5):
+    """T425 (1280 x 640 = 819200 columns, 8 levels) forward + VJP through the batched
+    entry: kernel time from the library's HIP-event timers (level-set pass, forward +
+    VJP, parameter reduction), after one untimed call."""
+    import torch
+    from jcm_amd import _native, synthetic
+    from jcm_amd.physics_interface import PhysicsTendency
+    from jcm_amd.serving import PhysicsServer, Request
+    st, fo, ge, da = to_device(synthetic.make_case(1280, 640, 8, seed=1, land=True), dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2)
+    cot = PhysicsTendency(*[torch.randn(st.temperature.shape, device=dev, generator=gen) for _ in range(4)])
+    server = PhysicsServer(phys, n_streams=1)
+    lib = _native.load()
+    server.run([Request(state=st, forcing=fo, geometry=ge, date=da, cotangent=cot)], realtime=False)
+    torch.cuda.synchronize()
+    lib.speedy_timer_enable(16 * reps)
+    for i in range(reps):
+        server.run([Request(state=st, forcing=fo, geometry=ge, date=da, cotangent=cot, rid=i + 1)], realtime=False)
+    torch.cuda.synchronize()
+    tm = _native.timer_read()
+    lib.speedy_timer_disable()
+    ncol = 1280 * 640
+    per_kind = {k: v[0] / reps for k, v in tm.items() if v[1]}
+    ms = sum(per_kind.values())
+    kx = 8
+    achieved = VJP_BYTES_PER_COL(kx) * ncol / (per_kind["fwd_vjp"] * 1e-3) / 1e9
+    return {"workload": "T425 1280x640 = 819200 columns x 8 levels, forward + VJP, batched entry (one request)",
+            "columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
+            "roofline": {"bound": "hbm", "kernel": "speedy::fwd_batch_kernel<8>", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": profile_traffic("fwd_batch_kernel_T425")}}
 
 
 def main():
@@ -225,6 +278,7 @@ def main():
         for _ in range(reps):
             g_vjp.graph.replay()
         torch.cuda.synchronize()
+        large_leg(vphys, dev, reps=3)
         # A copy of known size (torch.neg: reads and writes copy_mib MiB each) whose
         # FETCH_SIZE / WRITE_SIZE counters calibrate the counter units per byte.
         n = (args.copy_mib << 20) // 4
@@ -234,7 +288,7 @@ def main():
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
         print(f"profile run: {steps} forward and {steps} forward+VJP calls at T85 replayed from graphs of {G}; "
-              f"3 x neg of {args.copy_mib} MiB", flush=True)
+              f"4 batched T425 forward+VJP calls; 3 x neg of {args.copy_mib} MiB", flush=True)
         return
 
     # ---- timed region (headline): K forward steps replayed from graphs
@@ -256,11 +310,7 @@ def main():
     achieved = bytes_fwd / (ms_per_step * 1e-3) / 1e9
     bytes_vjp = VJP_BYTES_PER_COL(kx) * ncol
     achieved_vjp = bytes_vjp / (ms_vjp * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "fwd_kernel_T85_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    traffic = profile_traffic("fwd_kernel_T85")
 
     out = {
         "metric": METRIC, "value": value, "unit": "columns/s", "n_gpus": 1, "steps": steps,
@@ -270,13 +320,14 @@ def main():
                    "columns": ncol, "levels": kx, "pass": "fwd", "graph_calls": G},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "speedy::fwd_kernel<8, false, true, false>", "kernel_ms": ms_per_step,
+                     "kernel": "speedy::fwd_kernel<8, false, true, false, false>", "kernel_ms": ms_per_step,
                      "algorithmic_bytes_per_launch": bytes_fwd},
         "fwd_vjp": {"value": ncol / (ms_vjp * 1e-3), "unit": "columns/s", "ms_per_step": ms_vjp,
                     "launches_per_step": 2,
                     "roofline": {"bound": "hbm", "achieved": achieved_vjp, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": achieved_vjp / HBM_PEAK_GBS, "algorithmic_bytes_per_step": bytes_vjp,
-                                 "kernel": "speedy::fwd_kernel<8, false, true, true> + param_reduce_kernel"}},
+                                 "traffic": profile_traffic("fwd_vjp_kernel_T85"),
+                                 "kernel": "speedy::fwd_kernel<8, false, true, true, true> + param_reduce_kernel"}},
         "launches_per_call": launches,
     }
     # ---- plain per-call path (Python, no graphs)
@@ -285,6 +336,7 @@ def main():
     out["per_call"] = {"fwd_columns_per_s": ncol / (pc_f * 1e-3), "fwd_ms": pc_f,
                        "fwd_vjp_columns_per_s": ncol / (pc_v * 1e-3), "fwd_vjp_ms": pc_v,
                        "speedup_graph_over_per_call": {"fwd": pc_f / ms_per_step, "fwd_vjp": pc_v / ms_vjp}}
+    out["large"] = large_leg(vphys, dev)
     if not args.no_serving:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         out["serving"] = serving_runs(vphys, dev, args.serve_requests, rates)

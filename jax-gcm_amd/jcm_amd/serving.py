@@ -356,6 +356,14 @@ class PhysicsServer:
                 raise _native.SpeedyError("serving loop timed out")
         return self.done
 
+    def reset_stats(self):
+        """Forget completed requests and batch counts (after a warm-up run); the
+        workspaces, descriptors and member structs built so far are kept."""
+        self.done = []
+        self.n_batches = 0
+        self.batch_sizes = []
+        self.peak_bytes = self.in_flight_bytes
+
     # ------------------------------------------------------------ report
     def stats(self, requests: list | None = None) -> dict:
         reqs = requests if requests is not None else self.done
