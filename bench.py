@@ -241,7 +241,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the SPEEDY path shards by columns with no exchange but a level-set OR; see DESIGN.md)",
+              "(the SPEEDY path shards by columns with no exchange but a per-member level-set OR)",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST_SO = os.path.join(ROOT, "tests", "host_column", "libcolumn_host.so")
 
 # Tolerance of the parity tests: ||F_gpu - F_ref||_inf / ||F_ref||_inf per tendency field,
-# against the float32 oracle on the same inputs (BASELINE north star: about 1e-5 in FP32).
+# against the float32 oracle on the same inputs (about 1e-5 in FP32).
 REL_TOL = 1e-5
 
 
