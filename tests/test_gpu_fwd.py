@@ -191,6 +191,37 @@ def test_fwd_members_stacked(phys):
     assert S[0] != S[1]
 
 
+@pytest.mark.parametrize("nlon,nlat,nmember,kx", [(100, 50, 3, 8), (90, 45, 2, 8), (100, 50, 2, 16)])
+def test_fwd_ragged_members(phys, nlon, nlat, nmember, kx):
+    """Members whose column count is not a multiple of the 256-column workgroup (nor,
+    for 90 x 45, of the 64-lane wave): workgroups straddle two members, whose level
+    sets and float64 budgets must stay apart.  Every member matches its own oracle."""
+    import torch
+    phys.parameters = Parameters.default()
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, kx, seed=11, land=True, nmember=nmember, member_t_sigma=0.5)
+    q = np.array(st.specific_humidity)
+    q[1::2] *= np.float32(1.15)
+    case = (st.copy(specific_humidity=q), fo, ge, da)
+    tend, _ = phys.compute_tendencies(*case, budgets=True)
+    torch.cuda.synchronize()
+    S = phys.level_sets(nlon, nlat, nmember)
+    bud = phys.last_budgets.cpu().numpy()
+    assert phys.barrier_status(nlon, nlat, nmember) == 0
+    for m in range(nmember):
+        ref, _, S_ref = H.run_oracle(*case, member=m)
+        ref64, _, _ = H.run_oracle(*case, member=m, dtype=np.float64)
+        assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
+        for f in FIELDS:
+            ok, e32, e64, eref = H.parity(_np(getattr(tend, f)[m]), getattr(ref, f), getattr(ref64, f))
+            assert ok, f"member {m} {f}: {e32:.3e} (fp64 {e64:.2e}, oracle {eref:.2e})"
+        one = H.subgrid(case, np.arange(nlon), np.arange(nlat), m)
+        rb = H.oracle_budgets(one)
+        for i in (0, 1, 2, 3):
+            scale = max(abs(rb[2]), abs(rb[3]), 1e-12) if i else abs(rb[0])
+            assert abs(bud[m, i] - rb[i]) <= 1e-5 * scale + (1e-3 if i == 0 else 0.0), (m, i, bud[m, i], rb[i])
+    assert S[0] != S[1]
+
+
 def test_fwd_repeated_calls_reset_level_set(phys):
     """The level set of one call does not leak into the next (self-resetting workspace)."""
     import torch

@@ -87,6 +87,43 @@ def test_batched_results_equal_alone():
     print(f"batched == alone bit for bit: {identical}; batches {server.n_batches}", flush=True)
 
 
+def test_batched_ragged_members_equal_alone():
+    """Segments whose members do not fill whole workgroups (100 x 50 and 90 x 45
+    columns, 1-3 members, forward and forward + VJP) in one batch: each result equals
+    the same request called alone."""
+    import torch
+    from jcm_amd.serving import PhysicsServer, Request
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2))
+    specs = [((100, 50), 3, False), ((100, 50), 3, True), ((90, 45), 2, False), ((90, 45), 2, True),
+             ((64, 32), 1, True), ((100, 50), 1, False)]
+    reqs = []
+    for i, ((nx, ny), nm, vjp) in enumerate(specs):
+        st, fo, ge, da = _device_case(nx, ny, 40 + i, nm)
+        cot = None
+        if vjp:
+            cot = PhysicsTendency(*[torch.randn(st.temperature.shape, device=st.temperature.device) for _ in range(4)])
+        reqs.append(Request(state=st, forcing=fo, geometry=ge, date=da, cotangent=cot, rid=i))
+    server = PhysicsServer(phys, n_streams=1)
+    done = server.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    assert len(done) == len(reqs) and server.n_batches == 1
+    alone = SpeedyPhysics(parameters=phys.parameters)
+    for r in reqs:
+        if r.cotangent is None:
+            t, _ = alone.compute_tendencies(r.state, r.forcing, r.geometry, r.date)
+        else:
+            t, sg, pg = alone.compute_tendencies_vjp(r.state, r.forcing, r.geometry, r.date, r.cotangent)
+        torch.cuda.synchronize()
+        for f in ("u_wind", "v_wind", "temperature", "specific_humidity"):
+            _same(getattr(r.tendency, f), getattr(t, f), f"req {r.rid} {f}")
+        if r.cotangent is not None:
+            for f in ("u_wind", "v_wind", "temperature", "specific_humidity", "normalized_surface_pressure"):
+                _same(getattr(r.state_grads, f), getattr(sg, f), f"req {r.rid} grad {f}")
+            for k in pg:
+                _same(r.param_grads[k], pg[k], f"req {r.rid} {k}")
+
+
 def test_admission_respects_hbm_budget_and_overlaps():
     import torch
     from jcm_amd.serving import PhysicsServer

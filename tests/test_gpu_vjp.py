@@ -171,7 +171,7 @@ def test_vjp_finite_at_unit_cotangents(phys):
 _TWO_LAUNCH = "same shapes and non-batched two-launch path as the forward cases kept off the GPU (test_gpu_fwd)"
 
 
-@pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3),
+@pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3), (100, 50, 3), (90, 45, 2),
                                                pytest.param(256, 128, 8, marks=pytest.mark.skip(reason=_TWO_LAUNCH)),
                                                pytest.param(8, 4, 6, marks=pytest.mark.skip(reason=_TWO_LAUNCH))])
 def test_vjp_members_stacked(phys, nlon, nlat, nmember):
