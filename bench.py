@@ -204,9 +204,9 @@ def profile_traffic(name):
 
 
 def large_leg(phys, dev, reps=5):
-    """T425 (1280 x 640 = 819200 columns, 8 levels) forward + VJP through the batched
-    entry: kernel time from the library's HIP-event timers (level-set pass, forward +
-    VJP, parameter reduction), after one untimed call."""
+    """T425 (1280 x 640 = 819200 columns, 8 levels) forward + VJP, and the forward
+    alone, through the batched entry: kernel time from the library's HIP-event timers
+    (level-set pass, forward (+ VJP), parameter reduction), after one untimed call each."""
     import torch
     from jcm_amd import _native, synthetic
     from jcm_amd.physics_interface import PhysicsTendency
@@ -218,6 +218,7 @@ def large_leg(phys, dev, reps=5):
     server = PhysicsServer(phys, n_streams=1)
     lib = _native.load()
     server.run([Request(state=st, forcing=fo, geometry=ge, date=da, cotangent=cot)], realtime=False)
+    server.run([Request(state=st, forcing=fo, geometry=ge, date=da, rid=-1)], realtime=False)
     torch.cuda.synchronize()
     lib.speedy_timer_enable(16 * reps)
     for i in range(reps):
@@ -230,11 +231,26 @@ def large_leg(phys, dev, reps=5):
     ms = sum(per_kind.values())
     kx = 8
     achieved = VJP_BYTES_PER_COL(kx) * ncol / (per_kind["fwd_vjp"] * 1e-3) / 1e9
-    return {"workload": "T425 1280x640 = 819200 columns x 8 levels, forward + VJP, batched entry (one request)",
-            "columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
-            "roofline": {"bound": "hbm", "kernel": "speedy::fwd_batch_kernel<8>", "achieved": achieved,
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "traffic": profile_traffic("fwd_batch_kernel_T425")}}
+    out = {"workload": "T425 1280x640 = 819200 columns x 8 levels, forward + VJP, batched entry (one request)",
+           "columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
+           "roofline": {"bound": "hbm", "kernel": "speedy::fwd_batch_kernel<8>", "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": profile_traffic("fwd_batch_kernel_T425")}}
+    # the forward alone (forward-only batch kernel)
+    lib.speedy_timer_enable(16 * reps)
+    for i in range(reps):
+        server.run([Request(state=st, forcing=fo, geometry=ge, date=da, rid=100 + i)], realtime=False)
+    torch.cuda.synchronize()
+    tm = _native.timer_read()
+    lib.speedy_timer_disable()
+    per_kind = {k: v[0] / reps for k, v in tm.items() if v[1]}
+    ms = sum(per_kind.values())
+    achieved = FWD_BYTES_PER_COL(kx) * ncol / (per_kind["fwd"] * 1e-3) / 1e9
+    out["fwd"] = {"columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
+                  "roofline": {"bound": "hbm", "kernel": "speedy::fwd_only_batch_kernel<8>", "achieved": achieved,
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                               "traffic": profile_traffic("fwd_only_batch_kernel_T425")}}
+    return out
 
 
 def main():

@@ -651,6 +651,13 @@ __global__ __launch_bounds__(kBlock) void levelset_batch_kernel(const KArgs* seg
   levelset_body<KX>(segs[e >> kSegShift], e & kSegLocalMask, s_or);
 }
 
+// Forward-only segments of a batch: the forward without the adjoint's registers.
+template <int KX>
+__global__ __launch_bounds__(kBlock) void fwd_only_batch_kernel(const KArgs* segs, const uint32_t* table) {
+  const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
+  fwd_body<KX, false, false, false, true>(segs[e >> kSegShift], e & kSegLocalMask);
+}
+
 template <int KX>
 __global__ __launch_bounds__(kBlock) void fwd_batch_kernel(const KArgs* segs, const uint32_t* table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
@@ -1036,6 +1043,7 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
   uint32_t* t_ls = reinterpret_cast<uint32_t*>(h + L.ls);
   uint32_t* t_red = reinterpret_cast<uint32_t*>(h + L.red);
   uint32_t nb = 0, nl = 0, nr = 0, jmax = 1;
+  std::vector<uint8_t> is_vjp((size_t)nseg, 0u);
   for (int32_t i = 0; i < nseg; ++i) {
     const SpeedySegment& sg = segs[i];
     if (sg.fields.diag) return fail(SPEEDY_ERR_ARG, "speedy_fwd_batch: segments write no diagnostics");
@@ -1051,7 +1059,7 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
       return rc;
     a.adj = sg.adj;
     std::memcpy(&ha[i], &a, sizeof(KArgs));
-    for (uint32_t b = 0; b < a.nblk; ++b) t_blk[nb++] = ((uint32_t)i << kSegShift) | b;
+    is_vjp[i] = nset != 0;
     if (a.level_set < 0) {
       for (uint32_t b = 0; b < a.ls_blocks; ++b) t_ls[nl++] = ((uint32_t)i << kSegShift) | b;
       jmax = std::max(jmax, a.jmax);
@@ -1059,6 +1067,15 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
     if (nset || sg.fields.budget)
       for (uint32_t m = 0; m < a.nmember; ++m) t_red[nr++] = ((uint32_t)i << kSegShift) | m;
   }
+  // block table: the forward-only segments' blocks first, then the VJP segments'
+  for (int pass = 0; pass < 2; ++pass)
+    for (int32_t i = 0; i < nseg; ++i)
+      if (is_vjp[i] == pass)
+        for (uint32_t b = 0; b < ha[i].nblk; ++b) t_blk[nb++] = ((uint32_t)i << kSegShift) | b;
+  uint32_t nbf = 0;
+  for (int32_t i = 0; i < nseg; ++i)
+    if (!is_vjp[i]) nbf += ha[i].nblk;
+  const uint32_t nbv = nb - nbf;
   g_last_error.clear();
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipMemcpyAsync(desc, host_staging, L.total, hipMemcpyHostToDevice, s);
@@ -1071,16 +1088,24 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
   // One flag per level count: the three forward kernels share this lambda's
   // instantiation (same pointer types) but not their private segment sizes.
   static bool stack_ok[3] = {false, false, false};
-  auto launch = [&](auto ls_kernel, auto fwd_kernel_b, int kxi) {
-    if (!stack_ok[kxi] && ensure_stack(fwd_kernel_b) == hipSuccess) stack_ok[kxi] = true;
+  auto launch = [&](auto ls_kernel, auto fwd_only_b, auto fwd_kernel_b, int kxi) {
+    if (!stack_ok[kxi] && ensure_stack(fwd_kernel_b) == hipSuccess && ensure_stack(fwd_only_b) == hipSuccess)
+      stack_ok[kxi] = true;
     if (nl) {
       const int slot = timer_begin(s, 0);
       hipLaunchKernelGGL(ls_kernel, dim3(nl), dim3(kBlock), jmax * sizeof(uint32_t), s, d_args, d_ls);
       timer_end(s, slot);
     }
-    const int slot = timer_begin(s, 2);
-    hipLaunchKernelGGL(fwd_kernel_b, dim3(nb), dim3(kBlock), 0, s, d_args, d_blk);
-    timer_end(s, slot);
+    if (nbf) {
+      const int slot = timer_begin(s, 1);
+      hipLaunchKernelGGL(fwd_only_b, dim3(nbf), dim3(kBlock), 0, s, d_args, d_blk);
+      timer_end(s, slot);
+    }
+    if (nbv) {
+      const int slot = timer_begin(s, 2);
+      hipLaunchKernelGGL(fwd_kernel_b, dim3(nbv), dim3(kBlock), 0, s, d_args, d_blk + nbf);
+      timer_end(s, slot);
+    }
     if (nr) {
       const int slot2 = timer_begin(s, 3);
       hipLaunchKernelGGL(param_reduce_batch_kernel, dim3(nr), dim3(kBlock), 0, s, d_args, d_red);
@@ -1088,9 +1113,9 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
     }
   };
   switch (vert->kx) {
-    case 7: launch(levelset_batch_kernel<7>, fwd_batch_kernel<7>, 0); break;
-    case 8: launch(levelset_batch_kernel<8>, fwd_batch_kernel<8>, 1); break;
-    default: launch(levelset_batch_kernel<16>, fwd_batch_kernel<16>, 2); break;
+    case 7: launch(levelset_batch_kernel<7>, fwd_only_batch_kernel<7>, fwd_batch_kernel<7>, 0); break;
+    case 8: launch(levelset_batch_kernel<8>, fwd_only_batch_kernel<8>, fwd_batch_kernel<8>, 1); break;
+    default: launch(levelset_batch_kernel<16>, fwd_only_batch_kernel<16>, fwd_batch_kernel<16>, 2); break;
   }
   e = hipGetLastError();
   if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_batch: launch failed: ") + hipGetErrorString(e));
