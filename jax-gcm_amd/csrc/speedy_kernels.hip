@@ -669,6 +669,20 @@ __global__ __launch_bounds__(kBlock) void param_reduce_batch_kernel(const KArgs*
   member_reduce_body(segs[e >> kSegShift], e & kSegLocalMask);
 }
 
+// The call's arguments as a one-segment batch in the workspace: block 0 copies the
+// argument block word by word; every thread writes one table entry (the block-,
+// level-set-block- and member-tables in turn, each counting from 0).
+__global__ __launch_bounds__(kBlock) void seg_desc_kernel(KArgs a, KArgs* dst, uint32_t* tbl, uint32_t nblk,
+                                                          uint32_t nls, uint32_t nred) {
+  if (blockIdx.x == 0) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&a);
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+    for (uint32_t w = threadIdx.x; w < (uint32_t)(sizeof(KArgs) / 4); w += kBlock) d[w] = src[w];
+  }
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nblk + nls + nred) tbl[i] = i < nblk ? i : (i < nblk + nls ? i - nblk : i - nblk - nls);
+}
+
 __global__ void fill_zero_kernel(uint32_t* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -728,7 +742,7 @@ Grid make_grid(int64_t nlon, int64_t nlat, int64_t nmember) {
 }
 
 struct WsLayout {
-  size_t last_s, partial, prm, bud, total;
+  size_t last_s, partial, prm, bud, desc, tbl, total;
 };
 
 WsLayout ws_layout(const Grid& g, int32_t nmember) {
@@ -740,7 +754,10 @@ WsLayout ws_layout(const Grid& g, int32_t nmember) {
   const size_t nprm = g.ncol >= (uint32_t)kBlock ? (size_t)g.nblk * 2 * VJP_NPARAM : (size_t)g.ntot * VJP_NPARAM;
   L.bud = L.prm + align256(sizeof(float) * nprm);
   const size_t nbud = g.ncol >= (uint32_t)kBlock ? (size_t)g.nblk * 2 * kBudParts : (size_t)g.ntot * kBudParts;
-  L.total = L.bud + align256(sizeof(double) * nbud);
+  L.desc = L.bud + align256(sizeof(double) * nbud);
+  // one-segment batch descriptor and its block tables, for grids the fused launch cannot take
+  L.tbl = L.desc + align256(sizeof(KArgs));
+  L.total = L.tbl + align256(sizeof(uint32_t) * ((size_t)g.nblk + g.ls_blocks + (size_t)nmember));
   return L;
 }
 
@@ -837,11 +854,9 @@ template <int KX, bool DIAG, bool VJP, bool EXTRA>
 int launch_fwd(const KArgs& a, hipStream_t s) {
   static int64_t cap = -1;   // one GPU per process in this library's deployments
   if (cap < 0) {
-    if (VJP) {
-      hipError_t se = ensure_stack(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
-      if (se == hipSuccess) se = ensure_stack(fwd_kernel<KX, DIAG, false, VJP, true>);
-      if (se != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: stack limit: ") + hipGetErrorString(se));
-    }
+    hipError_t se = ensure_stack(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
+    if (se == hipSuccess) se = VJP ? ensure_stack(fwd_batch_kernel<KX>) : ensure_stack(fwd_only_batch_kernel<KX>);
+    if (se != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: stack limit: ") + hipGetErrorString(se));
     cap = coresident_blocks(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
   }
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -863,7 +878,46 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
       e = hipGetLastError();
     }
     timer_end(s, slot);
+  } else if (!DIAG) {
+    // Grids the fused launch cannot take (beyond one co-resident launch, members
+    // narrower than a workgroup, or a given level set) run as a one-segment batch
+    // through the batch kernels; the workspace holds the descriptor and tables.
+    if (a.nblk > kSegLocalMask || a.ls_blocks > kSegLocalMask)
+      return fail(SPEEDY_ERR_CAPACITY, "speedy: more than 2^20 workgroups (2^28 columns) in one call");
+    const WsLayout L = ws_layout(make_grid(a.ncol, 1, a.nmember), (int32_t)a.nmember);
+    char* w = reinterpret_cast<char*>(a.gen);
+    KArgs* d_args = reinterpret_cast<KArgs*>(w + L.desc);
+    uint32_t* tbl = reinterpret_cast<uint32_t*>(w + L.tbl);
+    const uint32_t nls = a.level_set < 0 ? a.ls_blocks : 0u;
+    const uint32_t nred = (VJP || a.f.budget) ? a.nmember : 0u;
+    const uint32_t n = a.nblk + nls + nred;
+    hipLaunchKernelGGL(seg_desc_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, d_args, tbl, a.nblk,
+                       nls, nred);
+    if (nls) {
+      const int slot = timer_begin(s, 0);
+      hipLaunchKernelGGL(levelset_batch_kernel<KX>, dim3(nls), dim3(kBlock), a.jmax * sizeof(uint32_t), s,
+                         (const KArgs*)d_args, (const uint32_t*)(tbl + a.nblk));
+      timer_end(s, slot);
+    }
+    const int slot = timer_begin(s, kind);
+    if (VJP)
+      hipLaunchKernelGGL(fwd_batch_kernel<KX>, dim3(a.nblk), dim3(kBlock), 0, s, (const KArgs*)d_args,
+                         (const uint32_t*)tbl);
+    else
+      hipLaunchKernelGGL(fwd_only_batch_kernel<KX>, dim3(a.nblk), dim3(kBlock), 0, s, (const KArgs*)d_args,
+                         (const uint32_t*)tbl);
+    timer_end(s, slot);
+    if (nred) {
+      const int slot2 = timer_begin(s, 3);
+      hipLaunchKernelGGL(param_reduce_batch_kernel, dim3(nred), dim3(kBlock), 0, s, (const KArgs*)d_args,
+                         (const uint32_t*)(tbl + a.nblk + nls));
+      timer_end(s, slot2);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: launch failed: ") + hipGetErrorString(e));
+    return SPEEDY_OK;
   } else {
+    // with diagnostics: the non-batched two-launch kernels
     int slot = -1;
     if (a.level_set < 0) {
       slot = timer_begin(s, 0);

@@ -222,6 +222,43 @@ def test_fwd_ragged_members(phys, nlon, nlat, nmember, kx):
     assert S[0] != S[1]
 
 
+@pytest.mark.parametrize("nlon,nlat,nmember,level_set", [(256, 128, 3, -1), (16, 8, 5, -1), (64, 32, 1, 0b0101010)])
+def test_fwd_one_segment_batch_route(nlon, nlat, nmember, level_set):
+    """Calls the fused launch cannot take run as a one-segment batch through the batch
+    kernels: beyond one co-resident launch (T85 x 3 members = 384 workgroups), members
+    narrower than a workgroup (16 x 8 columns), and a given level set.  Every member
+    matches its own oracle, with its own level set and float64 budgets."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()   # no diagnostics
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=13, land=True, nmember=nmember, member_t_sigma=0.5)
+    q = np.array(st.specific_humidity)
+    if nmember > 1:
+        q[1::2] *= np.float32(1.15)
+    case = (st.copy(specific_humidity=q), fo, ge, da)
+    tend, _ = phys.compute_tendencies(*case, level_set=level_set, budgets=True)
+    torch.cuda.synchronize()
+    S = phys.level_sets(nlon, nlat, nmember)
+    bud = phys.last_budgets.cpu().numpy()
+    for m in range(nmember):
+        mm = m if nmember > 1 else None
+        ref, _, S_ref = H.run_oracle(*case, member=mm, level_set=None if level_set < 0 else level_set)
+        ref64, _, _ = H.run_oracle(*case, member=mm, dtype=np.float64, level_set=None if level_set < 0 else level_set)
+        if level_set < 0:
+            assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
+        for f in FIELDS:
+            g = _np(getattr(tend, f)[m]) if nmember > 1 else _np(getattr(tend, f))
+            ok, e32, e64, eref = H.parity(g, getattr(ref, f), getattr(ref64, f))
+            assert ok, f"member {m} {f}: {e32:.3e} (fp64 {e64:.2e}, oracle {eref:.2e})"
+        if level_set < 0:
+            one = H.subgrid(case, np.arange(nlon), np.arange(nlat), mm)
+            rb = H.oracle_budgets(one)
+            wscale = max(abs(rb[2]), abs(rb[3]), 1e-12)
+            assert abs(bud[m, 0] - rb[0]) <= 1e-5 * abs(rb[0]) + 1e-3, (m, bud[m, 0], rb[0])
+            for i in (1, 2, 3):
+                assert abs(bud[m, i] - rb[i]) <= 1e-5 * wscale, (m, i, bud[m, i], rb[i])
+
+
 def test_fwd_repeated_calls_reset_level_set(phys):
     """The level set of one call does not leak into the next (self-resetting workspace)."""
     import torch

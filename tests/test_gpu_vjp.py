@@ -171,13 +171,14 @@ def test_vjp_finite_at_unit_cotangents(phys):
 _TWO_LAUNCH = "same shapes and non-batched two-launch path as the forward cases kept off the GPU (test_gpu_fwd)"
 
 
-@pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3), (100, 50, 3), (90, 45, 2),
+@pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3), (100, 50, 3), (90, 45, 2), (256, 128, 3), (16, 8, 5),
                                                pytest.param(256, 128, 8, marks=pytest.mark.skip(reason=_TWO_LAUNCH)),
                                                pytest.param(8, 4, 6, marks=pytest.mark.skip(reason=_TWO_LAUNCH))])
 def test_vjp_members_stacked(phys, nlon, nlat, nmember):
-    """Stacked members (fused path; beyond one co-resident launch; narrower than a
-    workgroup): each member's gradients equal its own float64 adjoint, with its
-    own level set and its own parameter-gradient sum."""
+    """Stacked members (fused path; members straddling workgroups; beyond one
+    co-resident launch and narrower than a workgroup, which run as a one-segment
+    batch): each member's gradients equal its own float64 adjoint, with its own
+    level set and its own parameter-gradient sum."""
     st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=9, land=True, nmember=nmember, member_t_sigma=1.0)
     q = st.specific_humidity.copy()
     q[1::2] *= np.float32(1.12)
