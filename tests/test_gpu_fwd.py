@@ -305,14 +305,15 @@ def test_fwd_convection_fortran_fixture(phys):
     np.testing.assert_allclose(_np(tend.specific_humidity), qt, atol=1e-4, rtol=0)
 
 
-@pytest.mark.parametrize("nmember", [1, 3])
-def test_fwd_graph_replay_matches_eager(nmember):
-    """Calls captured in a HIP graph (plain launch of the fused kernel) give, on every
-    replay, exactly the eager call's tendencies; eager work runs between replays."""
+@pytest.mark.parametrize("nlon,nlat,nmember", [(128, 64, 1), (128, 64, 3), (256, 128, 3)])
+def test_fwd_graph_replay_matches_eager(nlon, nlat, nmember):
+    """Calls captured in a HIP graph (a plain launch of the fused kernel, or at T85 x 3
+    members the one-segment batch with its descriptor kernel) give, on every replay,
+    exactly the eager call's tendencies; eager work runs between replays."""
     import torch
     from jcm_amd.speedy_physics import SpeedyPhysics
     phys = SpeedyPhysics()
-    st, fo, ge, da = synthetic.make_case(128, 64, 8, seed=4, land=True, nmember=nmember, member_t_sigma=1.0)
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=4, land=True, nmember=nmember, member_t_sigma=1.0)
     st = st.copy(specific_humidity=(st.specific_humidity * np.float32(1.1)).astype(np.float32))
     dev = phys.device
     st = st.copy(**{k: torch.as_tensor(np.ascontiguousarray(getattr(st, k)), device=dev)
@@ -320,7 +321,7 @@ def test_fwd_graph_replay_matches_eager(nmember):
                               "normalized_surface_pressure")})
     eager, _ = phys.compute_tendencies(st, fo, ge, da)
     eager = [getattr(eager, f).clone() for f in FIELDS]
-    S_eager = phys.level_sets(128, 64, nmember)
+    S_eager = phys.level_sets(nlon, nlat, nmember)
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
@@ -340,8 +341,8 @@ def test_fwd_graph_replay_matches_eager(nmember):
         for o in outs:
             for f, e in zip(FIELDS, eager):
                 assert torch.equal(getattr(o, f), e), (rep, f)
-        assert phys.level_sets(128, 64, nmember) == S_eager
-        assert phys.barrier_status(128, 64, nmember) == 0
+        assert phys.level_sets(nlon, nlat, nmember) == S_eager
+        assert phys.barrier_status(nlon, nlat, nmember) == 0
 
 
 TWO_LAUNCH_SKIP = ("GPU memory faults were reported at the first synchronize of this test twice (T85 x 8 members, "
