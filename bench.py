@@ -203,6 +203,17 @@ def profile_traffic(name):
         return json.load(fh).get("bytes_per_launch")
 
 
+def profile_valu(kernel):
+    """rocprofv3 VALUBusy / VALUUtilization (percent) of a kernel from the committed
+    summary profiles/speedy_kernels_valu.json, or None."""
+    path = os.path.join(ROOT, "profiles", "speedy_kernels_valu.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        k = json.load(fh)["kernels"].get(kernel)
+    return None if k is None else {"valu_busy_pct": k["VALUBusy"], "valu_utilization_pct": k["VALUUtilization"]}
+
+
 def large_leg(phys, dev, reps=5):
     """T425 (1280 x 640 = 819200 columns, 8 levels) forward + VJP, and the forward
     alone, through the batched entry: kernel time from the library's HIP-event timers
@@ -235,7 +246,8 @@ def large_leg(phys, dev, reps=5):
            "columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
            "roofline": {"bound": "hbm", "kernel": "speedy::fwd_batch_kernel<8>", "achieved": achieved,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": profile_traffic("fwd_batch_kernel_T425")}}
+                        "traffic": profile_traffic("fwd_batch_kernel_T425"),
+                        "valu": profile_valu("speedy::fwd_batch_kernel<8>")}}
     # the forward alone (forward-only batch kernel)
     lib.speedy_timer_enable(16 * reps)
     for i in range(reps):
@@ -249,7 +261,8 @@ def large_leg(phys, dev, reps=5):
     out["fwd"] = {"columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
                   "roofline": {"bound": "hbm", "kernel": "speedy::fwd_only_batch_kernel<8>", "achieved": achieved,
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                               "traffic": profile_traffic("fwd_only_batch_kernel_T425")}}
+                               "traffic": profile_traffic("fwd_only_batch_kernel_T425"),
+                               "valu": profile_valu("speedy::fwd_only_batch_kernel<8>")}}
     return out
 
 
@@ -337,12 +350,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "speedy::fwd_kernel<8, false, true, false, false>", "kernel_ms": ms_per_step,
+                     "valu": profile_valu("speedy::fwd_kernel<8, false, true, false, false>"),
                      "algorithmic_bytes_per_launch": bytes_fwd},
         "fwd_vjp": {"value": ncol / (ms_vjp * 1e-3), "unit": "columns/s", "ms_per_step": ms_vjp,
                     "launches_per_step": 2,
                     "roofline": {"bound": "hbm", "achieved": achieved_vjp, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": achieved_vjp / HBM_PEAK_GBS, "algorithmic_bytes_per_step": bytes_vjp,
                                  "traffic": profile_traffic("fwd_vjp_kernel_T85"),
+                                 "valu": profile_valu("speedy::fwd_kernel<8, false, true, true, true>"),
                                  "kernel": "speedy::fwd_kernel<8, false, true, true, true> + param_reduce_kernel"}},
         "launches_per_call": launches,
     }
