@@ -166,7 +166,10 @@ int speedy_diag_names(int32_t kx, char *buf, size_t buf_len);
  * (speedy_physics.py:88-127) for nmember stacked states.
  * level_set < 0: the Q1 cloud-top level set (shortwave_radiation.py:63-70) is
  * computed per member over all its columns; >= 0: used as given (bit k =
- * level k), for parity tests.  Stream-ordered; no allocation. */
+ * level k), for parity tests.  A grid whose workgroups fit on the GPU at once
+ * runs as one cooperative launch; other grids (and a given level set) run as a
+ * one-segment batch, described in the workspace, in two to four launches.
+ * Stream-ordered and safe to capture in a HIP graph; no allocation. */
 int speedy_fwd(const SpeedyFields *fields, const SpeedyVertical *vert, const SpeedyParams *params,
                const SpeedyDate *date, int32_t nlon, int32_t nlat, int32_t nmember, uint32_t flags,
                uint32_t term_mask, int32_t level_set, void *workspace, size_t workspace_bytes,

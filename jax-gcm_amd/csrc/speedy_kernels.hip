@@ -16,10 +16,13 @@
 //     slots of the blocks covering its member until every tag is current.
 //     Block 0 then advances the generation.  No read-modify-write atomics:
 //     contended same-address atomics cost microseconds per call on this GPU.
-//   two-kernel (larger grids): levelset_kernel, a fixed number of blocks each
-//     covering a contiguous chunk of columns, writes per-(chunk, member) ORs;
-//     fwd_kernel ORs the few chunks of its member, recomputes phase A and does
-//     phase B.
+//   two launches (larger grids, members narrower than a block, a given S, and
+//     every batch): a level-set kernel, a fixed number of blocks each covering a
+//     contiguous chunk of columns, writes per-(chunk, member) ORs; the forward
+//     kernel ORs the few chunks of its member, recomputes phase A and does
+//     phase B.  Single calls without diagnostics run this as a one-segment batch
+//     (seg_desc_kernel writes the descriptor into the workspace); calls with
+//     diagnostics use levelset_kernel and fwd_kernel<..., FUSED = false, ...>.
 //
 // One thread per column; the column lives in registers.  Accesses are
 // [field][level][column] with consecutive lanes on consecutive columns
