@@ -26,6 +26,11 @@ Quirks of the reference that are reproduced on purpose are marked ``Q<n>``
 
 Parity status of each function (how it is pinned) is listed in
 ``PINNED_BY`` at the bottom of this file and checked by tests/test_oracle_*.
+No reference vector exists for the composed call: ``compute_tendencies`` (the
+eleven terms in sequence, Q1 with heterogeneous cloud tops and the Q2 hand-off
+in composition) is parity unpinned by the reference; it is checked by the
+pinned terms it composes, the column water budget identity and a brute-force
+emulation of the Q1 scatter, and only an import of the reference could pin it.
 """
 from __future__ import annotations
 
