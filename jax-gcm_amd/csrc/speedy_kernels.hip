@@ -235,7 +235,19 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
 }
 
 // ---------------------------------------------------------------- level set S
+// OR over the wave of a word of level bits (bits 0..15; KX <= 16).  BALLOT: one
+// ballot per bit, no cross-lane data movement or LDS round trip (the forward
+// kernels).  Otherwise a butterfly of lane shuffles: the forward + VJP kernels
+// keep it, because their register allocation (several KB of spills per lane at
+// 16 levels) changed with the ballot form and gave wrong gradients on gfx950.
+template <bool BALLOT>
 __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+  if (BALLOT) {
+    uint32_t r = 0u;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r |= __ballot((v >> k) & 1u) != 0ull ? (1u << k) : 0u;
+    return r;
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
   return v;
@@ -250,7 +262,8 @@ __device__ __forceinline__ uint32_t clip_bit(int icl, int kx) {
 // Fused path: OR of the level bits of member m over the published words of the
 // blocks covering it, polled until every word carries `tag` (one wave).
 // Bounded: after kSpinLimit polls it flags the error word and returns.
-__device__ uint32_t collect_fused(const KArgs& a, uint32_t tag, uint32_t m) {
+template <bool BALLOT>
+__device__ __forceinline__ uint32_t collect_fused_t(const KArgs& a, uint32_t tag, uint32_t m) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t b0 = (m * a.ncol) / kBlock, b1 = (m * a.ncol + a.ncol - 1u) / kBlock;
   for (uint32_t spins = 0;; ++spins) {
@@ -262,22 +275,45 @@ __device__ uint32_t collect_fused(const KArgs& a, uint32_t tag, uint32_t m) {
       stale |= (w >> 16) != tag;
       acc |= w & 0xffffu;
     }
-    if (!__any(stale)) return wave_or(acc);
+    if (!__any(stale)) return wave_or<BALLOT>(acc);
     if (spins >= kSpinLimit) {
       if (lane == 0) __hip_atomic_fetch_or(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return wave_or(acc);
+      return wave_or<BALLOT>(acc);
     }
     __builtin_amdgcn_s_sleep(1);
   }
 }
 
-// Two-kernel path: OR over the chunks of levelset_kernel that cover member m.
+// The forward + VJP kernels' form (an outlined call, shuffle OR).
+__device__ uint32_t collect_fused(const KArgs& a, uint32_t tag, uint32_t m) { return collect_fused_t<false>(a, tag, m); }
+
+// Two-kernel path: OR over the chunks of levelset_kernel that cover member m, by
+// one wave (the forward + VJP kernels' form).
 __device__ uint32_t collect_chunks(const KArgs& a, uint32_t m) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
   uint32_t acc = 0;
   for (uint32_t b = b0 + lane; b <= b1; b += 64u) acc |= a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
-  return wave_or(acc);
+  return wave_or<false>(acc);
+}
+
+// Two-kernel path: member m's word of level-set chunk b (levelset_kernel output).
+__device__ __forceinline__ uint32_t chunk_word(const KArgs& a, uint32_t b, uint32_t m) {
+  return a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
+}
+
+// Two-kernel path, member wider than a block: this thread's share of the OR over
+// the chunks covering member m, kChunkLoads words per thread.  Indexes past the
+// member's last chunk are clamped to it (OR is idempotent), so the loads carry
+// no branches and issue together.  A member spans at most ls_blocks <= 1024 =
+// kChunkLoads * kBlock chunks (make_grid).
+constexpr uint32_t kChunkLoads = 4;
+__device__ __forceinline__ uint32_t chunk_share(const KArgs& a, uint32_t m) {
+  const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
+  uint32_t acc = 0u;
+#pragma unroll
+  for (uint32_t r = 0; r < kChunkLoads; ++r) acc |= chunk_word(a, min(b0 + threadIdx.x + r * kBlock, b1), m);
+  return acc;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -451,6 +487,21 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   uint32_t tag = 0u;
   if (FUSED && need_s && threadIdx.x < 64u)
     tag = gen_tag(__hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  // Two-kernel path: the level-set chunk words are loaded first too, so their
+  // round trip overlaps the state's.  A member wider than a block: each thread
+  // ORs its share of the chunks of the block's (at most two) members; narrower:
+  // the <= 2 chunks covering the thread's own member.
+  const bool wide = a.ncol >= (uint32_t)kBlock;   // a block spans at most two members
+  uint32_t lw0 = 0u, lw1 = 0u;
+  if (!FUSED && !VJP && need_s) {
+    if (wide) {
+      lw0 = chunk_share(a, m_first);
+      if (nm > 1u) lw1 = chunk_share(a, m_first + 1u);
+    } else {
+      const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
+      lw0 = chunk_word(a, b0, m) | chunk_word(a, b1, m);
+    }
+  }
   ColumnIn<float, KX> col;
   load_state<KX, EXTRA>(a, m, c, col);
   if (!VJP) {
@@ -464,7 +515,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
     // member slot (a fused block spans at most two members), then one word per slot
     __shared__ uint32_t s_wor[kBlock / 64][2];
     const uint32_t bit = active ? clip_bit(pa.icltop, KX) : 0u;
-    const uint32_t o0 = wave_or(m == m_first ? bit : 0u), o1 = wave_or(m != m_first ? bit : 0u);
+    const uint32_t o0 = wave_or<!VJP>(m == m_first ? bit : 0u), o1 = wave_or<!VJP>(m != m_first ? bit : 0u);
     if ((threadIdx.x & 63u) == 0) { s_wor[threadIdx.x >> 6][0] = o0; s_wor[threadIdx.x >> 6][1] = o1; }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -479,9 +530,9 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   PhaseB<float, KX> pb;
   phase_b1<float, KX>(col, pa, a.p, a.v, a.u, pb);
   uint32_t S = (uint32_t)a.level_set;
-  const bool wide = a.ncol >= (uint32_t)kBlock;   // a block spans at most two members
-  if (need_s) {
-    if (threadIdx.x < 64u) {   // wave 0 gathers S for the block's member(s)
+  if (VJP && need_s) {
+    // the forward + VJP kernels: wave 0 gathers S for the block's member(s)
+    if (threadIdx.x < 64u) {
       for (uint32_t j = 0; wide && j < nm; ++j) {
         const uint32_t v = FUSED ? collect_fused(a, tag, m_first + j) : collect_chunks(a, m_first + j);
         if (threadIdx.x == 0) s_s[j] = v;
@@ -507,6 +558,41 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
       const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
       for (uint32_t b = b0; b <= b1; ++b) S |= a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
     }
+  } else if (FUSED && need_s) {
+    if (threadIdx.x < 64u) {   // wave 0 gathers S for the block's member(s)
+      for (uint32_t j = 0; j < nm; ++j) {
+        const uint32_t v = collect_fused_t<true>(a, tag, m_first + j);
+        if (threadIdx.x == 0) s_s[j] = v;
+      }
+      if (bid == 0) {
+        // every member's S for speedy_read_level_sets; this also waits for every
+        // block's word, i.e. every block has read the generation
+        for (uint32_t mm = 0; mm < a.nmember; ++mm) {
+          const uint32_t v = collect_fused_t<true>(a, tag, mm);
+          if (threadIdx.x == 0) a.last_s[mm] = v;
+        }
+        if (threadIdx.x == 0) {
+          const uint32_t g = __hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(a.gen, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    __syncthreads();
+    S = s_s[m - m_first];
+  } else if (need_s) {
+    if (wide) {   // the block's OR per member slot
+      __shared__ uint32_t s_lw[kBlock / 64][2];
+      const uint32_t o0 = wave_or<true>(lw0), o1 = wave_or<true>(lw1);
+      if ((threadIdx.x & 63u) == 0) { s_lw[threadIdx.x >> 6][0] = o0; s_lw[threadIdx.x >> 6][1] = o1; }
+      __syncthreads();
+      S = 0u;
+#pragma unroll
+      for (int i = 0; i < kBlock / 64; ++i) S |= s_lw[i][m - m_first];
+    } else {
+      S = lw0;
+    }
+    // every member's S for speedy_read_level_sets, from its first column
+    if (active && c == 0u) a.last_s[m] = S;
   }
   // The forward + VJP kernels keep the zonal loads and all output stores here:
   // moving them earlier changed their register allocation and, at 16 levels,
@@ -800,6 +886,8 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   if (ntot >= ((int64_t)1 << 31) - kBlock) return fail(SPEEDY_ERR_CAPACITY, "speedy: more than 2^31 columns per call");
   const Grid gr = make_grid(nlon, nlat, nmember);
   if (gr.jmax > kMaxJ) return fail(SPEEDY_ERR_CAPACITY, "speedy: too many members per column chunk");
+  if (gr.ls_blocks > kChunkLoads * kBlock)   // chunk_share's reach (make_grid keeps it <= 1024)
+    return fail(SPEEDY_ERR_CAPACITY, "speedy: too many level-set chunks");
   const WsLayout L = ws_layout(gr, nmember);
   if (!workspace) return fail(SPEEDY_ERR_WORKSPACE, "speedy: workspace is NULL");
   if (workspace_bytes < L.total)
