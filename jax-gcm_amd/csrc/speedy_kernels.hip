@@ -235,18 +235,23 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
 }
 
 // ---------------------------------------------------------------- level set S
-// OR over the wave of a word of level bits (bits 0..15; KX <= 16).  BALLOT: one
-// ballot per bit, no cross-lane data movement or LDS round trip (the forward
-// kernels).  Otherwise a butterfly of lane shuffles: the forward + VJP kernels
-// keep it, because their register allocation (several KB of spills per lane at
-// 16 levels) changed with the ballot form and gave wrong gradients on gfx950.
-template <bool BALLOT>
+// OR over the 64 lanes of a wave (all active).  DPP: six data-parallel-primitive
+// ORs within the register file (quad swaps, half-row and row mirrors give every
+// lane its row's OR; the row-15 and row-31 broadcasts fold rows into row 3), then
+// lane 63 is read: no LDS round trip (the forward kernels).  Otherwise a
+// butterfly of lane shuffles: the forward + VJP kernels keep it, because their
+// register allocation (several KB of spills per lane at 16 levels) changed with
+// another form and gave wrong gradients on gfx950.
+template <bool DPP>
 __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-  if (BALLOT) {
-    uint32_t r = 0u;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) r |= __ballot((v >> k) & 1u) != 0ull ? (1u << k) : 0u;
-    return r;
+  if (DPP) {
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x140, 0xF, 0xF, false);   // row_mirror
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);   // row_bcast:15 into rows 1, 3
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);   // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
@@ -262,7 +267,7 @@ __device__ __forceinline__ uint32_t clip_bit(int icl, int kx) {
 // Fused path: OR of the level bits of member m over the published words of the
 // blocks covering it, polled until every word carries `tag` (one wave).
 // Bounded: after kSpinLimit polls it flags the error word and returns.
-template <bool BALLOT>
+template <bool DPP>
 __device__ __forceinline__ uint32_t collect_fused_t(const KArgs& a, uint32_t tag, uint32_t m) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t b0 = (m * a.ncol) / kBlock, b1 = (m * a.ncol + a.ncol - 1u) / kBlock;
@@ -275,10 +280,10 @@ __device__ __forceinline__ uint32_t collect_fused_t(const KArgs& a, uint32_t tag
       stale |= (w >> 16) != tag;
       acc |= w & 0xffffu;
     }
-    if (!__any(stale)) return wave_or<BALLOT>(acc);
+    if (!__any(stale)) return wave_or<DPP>(acc);
     if (spins >= kSpinLimit) {
       if (lane == 0) __hip_atomic_fetch_or(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return wave_or<BALLOT>(acc);
+      return wave_or<DPP>(acc);
     }
     __builtin_amdgcn_s_sleep(1);
   }
