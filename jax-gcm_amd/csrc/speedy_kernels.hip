@@ -398,6 +398,26 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// One DPP step of a float sum: x plus the value DPP control CTRL selects (lanes
+// outside ROWS keep x).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_add(float x) {
+  return x + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, ROWS, 0xF, false));
+}
+
+// Float sum over the 64 lanes of a wave (all active) in a fixed order, by the
+// DPP steps of wave_or (every lane holds its row's sum after the row mirror; the
+// row broadcasts add rows 0-2 into row 3); the total is read from lane 63.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v = dpp_add<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xF>(v);   // row_half_mirror
+  v = dpp_add<0x140, 0xF>(v);   // row_mirror
+  v = dpp_add<0x142, 0xA>(v);   // row_bcast:15 into rows 1, 3
+  v = dpp_add<0x143, 0xC>(v);   // row_bcast:31 into rows 2, 3
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
 // Parameter gradients of this block: a fixed-order tree sum per member slot
 // (deterministic), written to prm_part[block][slot][5].  Members narrower than
 // a block write per-column values instead (prm_part[column][5]).
@@ -648,8 +668,10 @@ __device__ __forceinline__ void budget_reduce_body(const KArgs& a, uint32_t m) {
   double acc[kBudParts];
 #pragma unroll
   for (int i = 0; i < kBudParts; ++i) acc[i] = 0.0;
+  // (unrolled: several iterations' loads in flight; the order of the sums is the loop's)
   if (a.ncol >= (uint32_t)kBlock) {
     const uint32_t b0 = (m * a.ncol) / kBlock, b1 = (m * a.ncol + a.ncol - 1u) / kBlock;
+#pragma unroll 4
     for (uint32_t b = b0 + threadIdx.x; b <= b1; b += kBlock) {
       const uint32_t j = m - divq(b * kBlock, a.dcol);
 #pragma unroll
@@ -659,6 +681,7 @@ __device__ __forceinline__ void budget_reduce_body(const KArgs& a, uint32_t m) {
       }
     }
   } else {
+#pragma unroll 4
     for (uint32_t cc = threadIdx.x; cc < a.ncol; cc += kBlock)
 #pragma unroll
       for (int i = 0; i < kBudParts; ++i) {
@@ -693,14 +716,17 @@ __device__ __forceinline__ void param_reduce_body(const KArgs& a, uint32_t m) {
   float acc[VJP_NPARAM];
 #pragma unroll
   for (int i = 0; i < VJP_NPARAM; ++i) acc[i] = 0.0f;
+  // (unrolled: several iterations' loads in flight; the order of the sums is the loop's)
   if (a.ncol >= (uint32_t)kBlock) {
     const uint32_t b0 = (m * a.ncol) / kBlock, b1 = (m * a.ncol + a.ncol - 1u) / kBlock;
+#pragma unroll 4
     for (uint32_t b = b0 + threadIdx.x; b <= b1; b += kBlock) {
       const uint32_t j = m - divq(b * kBlock, a.dcol);
 #pragma unroll
       for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += a.prm_part[((size_t)b * 2 + j) * VJP_NPARAM + i];
     }
   } else {
+#pragma unroll 4
     for (uint32_t cc = threadIdx.x; cc < a.ncol; cc += kBlock)
 #pragma unroll
       for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += a.prm_part[((size_t)m * a.ncol + cc) * VJP_NPARAM + i];
@@ -709,7 +735,7 @@ __device__ __forceinline__ void param_reduce_body(const KArgs& a, uint32_t m) {
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
 #pragma unroll
   for (int i = 0; i < VJP_NPARAM; ++i) {
-    const float v = wave_sum(acc[i]);
+    const float v = wave_sum_dpp(acc[i]);
     if (lane == 0) s_red[wave][i] = v;
   }
   __syncthreads();
