@@ -120,6 +120,11 @@ class Geometry:
     def ncol(self) -> int:
         return self.nodal_shape[1] * self.nodal_shape[2]
 
+    def vertical_key(self) -> tuple:
+        """Every value to_abi packs, as bytes: equal keys give equal ABI structs."""
+        return (self.kx,) + tuple(np.asarray(getattr(self, n)).tobytes()
+                                  for n in ("hsg", "fsg", "dhs", "sigl", "grdsig", "grdscp", "wvi"))
+
     def to_abi(self) -> _native.SpeedyVertical:
         v = _native.SpeedyVertical()
         kx = self.kx

@@ -102,6 +102,10 @@ class VerticalDiffusionParameters:   # params.py:211-232
     segrad: float = 0.1
 
 
+# The parameter groups in the order of the C struct (speedy_abi.h SpeedyParams).
+_GROUPS = ("convection", "condensation", "shortwave_radiation", "mod_radcon", "surface_flux",
+           "vertical_diffusion", "forcing")
+
 # The five scalars the fused VJP differentiates, in the order of g_params.
 VJP_PARAMS = (("convection", "rhbl"), ("convection", "trcnv"), ("condensation", "rhlsc"),
               ("forcing", "co2_scale"), ("surface_flux", "drag_scale"))
@@ -128,10 +132,14 @@ class Parameters:                    # params.py:234-294
             setattr(new, g, dataclasses.replace(getattr(self, g), **changes))
         return new
 
+    def abi_key(self) -> tuple:
+        """Every value to_abi packs, group by group: equal keys give equal ABI structs
+        (a cheap test for in-place changes between calls)."""
+        return tuple(tuple(vars(getattr(self, grp)).values()) for grp in _GROUPS)
+
     def to_abi(self) -> _native.SpeedyParams:
         p = _native.SpeedyParams()
-        for grp in ("convection", "condensation", "shortwave_radiation", "mod_radcon", "surface_flux",
-                    "vertical_diffusion", "forcing"):
+        for grp in _GROUPS:
             for f in dataclasses.fields(getattr(self, grp)):
                 v = getattr(getattr(self, grp), f.name)
                 setattr(p, f.name, int(v) if isinstance(v, bool) or f.name == "co2_year_ref" else float(v))

@@ -101,6 +101,8 @@ class SpeedyPhysics(Physics):
         self._consts: dict = {}
         self._abi_params = None
         self._abi_params_src = None
+        self._verts: dict = {}
+        self._fg: dict = {}
         self.last_budgets = None
 
     # ------------------------------------------------------------------ helpers
@@ -109,10 +111,22 @@ class SpeedyPhysics(Physics):
             raise _native.SpeedyError("SpeedyPhysics needs a ROCm GPU; there is no CPU path")
 
     def _params(self) -> _native.SpeedyParams:
-        if self._abi_params is None or self._abi_params_src != repr(self.parameters):
+        """parameters.to_abi(), rebuilt only when a value changed (also in place)."""
+        key = self.parameters.abi_key()
+        if self._abi_params is None or self._abi_params_src != key:
             self._abi_params = self.parameters.to_abi()
-            self._abi_params_src = repr(self.parameters)
+            self._abi_params_src = key
         return self._abi_params
+
+    def _vertical(self, geometry: Geometry) -> _native.SpeedyVertical:
+        """geometry.to_abi(), rebuilt only for another geometry or changed levels."""
+        key = geometry.vertical_key()
+        hit = self._verts.get(id(geometry))
+        if hit is None or hit[0] is not geometry or hit[1] != key:
+            if len(self._verts) > 64:
+                self._verts.clear()
+            hit = self._verts[id(geometry)] = (geometry, key, geometry.to_abi())
+        return hit[2]
 
     def workspace(self, nlon: int, nlat: int, nmember: int) -> torch.Tensor:
         key = (nlon, nlat, nmember)
@@ -164,24 +178,41 @@ class SpeedyPhysics(Physics):
             raise ValueError("normalized_surface_pressure has the wrong size")
         keep.append(ps)
         f.ps = ps.data_ptr()
-        day = date.model_day()
-        fmap = dict(zip(FORCING_FIELDS, ("alb0", "sice_am", "snowc_am", "soilw_am", "stl_am", "sst")))
-        for src, dst in fmap.items():
-            arr = getattr(forcing, src)
-            if isinstance(arr, np.ndarray):
-                arr = day_slice(arr, day)
-                if arr.shape != (ix, il):
-                    arr = np.broadcast_to(arr, (ix, il))
-            elif arr.dim() > 2:
-                arr = arr[:, :, min(max(day, 0), arr.shape[2] - 1)]
-            x = self._const(arr)
+        for dst, x in self._forcing_fields(forcing, geometry, date.model_day(), ix, il):
             keep.append(x)
             setattr(f, dst, x.data_ptr())
-        for src in ("fmask", "orog", "phis0", "sia", "coa"):
-            x = self._const(getattr(geometry, src))
-            keep.append(x)
-            setattr(f, src, x.data_ptr())
         return f, t, nmember
+
+    def _forcing_fields(self, forcing: ForcingData, geometry: Geometry, day: int, ix: int, il: int):
+        """(field, device tensor) of the forcing at `day` and the geometry.  Cached for
+        the same forcing and geometry objects and day when every array is used by
+        identity (NumPy arrays through _const's cache, device tensors as they are);
+        day slices, broadcasts and host tensors are copied again on every call."""
+        key = (id(forcing), id(geometry), day, ix, il)
+        hit = self._fg.get(key)
+        if hit is not None and hit[0] is forcing and hit[1] is geometry:
+            return hit[2]
+        out, by_identity = [], True
+        fmap = dict(zip(FORCING_FIELDS, ("alb0", "sice_am", "snowc_am", "soilw_am", "stl_am", "sst")))
+        srcs = [(dst, getattr(forcing, src)) for src, dst in fmap.items()]
+        srcs += [(src, getattr(geometry, src)) for src in ("fmask", "orog", "phis0", "sia", "coa")]
+        for i, (dst, src) in enumerate(srcs):
+            arr = src
+            if i < len(fmap):
+                if isinstance(arr, np.ndarray):
+                    arr = day_slice(arr, day)
+                    if arr.shape != (ix, il):
+                        arr = np.broadcast_to(arr, (ix, il))
+                elif arr.dim() > 2:
+                    arr = arr[:, :, min(max(day, 0), arr.shape[2] - 1)]
+            x = self._const(arr)
+            by_identity &= arr is src and (isinstance(arr, np.ndarray) or x is arr)
+            out.append((dst, x))
+        if by_identity:
+            if len(self._fg) > 64:
+                self._fg.clear()
+            self._fg[key] = (forcing, geometry, out)
+        return out
 
     # ------------------------------------------------------------------ API
     def compute_tendencies(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry,
@@ -216,7 +247,7 @@ class SpeedyPhysics(Physics):
             f.budget = self.last_budgets.data_ptr()
         ws = self.workspace(ix, il, nmember)
         flags = (_native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0) | (VERIFY_FLAGS if verify else 0)
-        rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(geometry.to_abi()), ctypes.byref(self._params()),
+        rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(self._vertical(geometry)), ctypes.byref(self._params()),
                                   ctypes.byref(date.to_abi()), ix, il, nmember, flags, term_mask, level_set,
                                   ws.data_ptr(), ws.numel(), self._stream())
         _native.check(rc, "speedy_fwd")
@@ -275,7 +306,7 @@ class SpeedyPhysics(Physics):
         f, adj, outputs, (ix, il, nmember), flags = self.prepare_vjp(state, forcing, geometry, date, cotangent, keep)
         flags |= VERIFY_FLAGS if verify else 0
         ws = self.workspace(ix, il, nmember)
-        rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(geometry.to_abi()),
+        rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(self._vertical(geometry)),
                                       ctypes.byref(self._params()), ctypes.byref(date.to_abi()), ix, il, nmember,
                                       flags, level_set, ws.data_ptr(), ws.numel(), self._stream())
         _native.check(rc, "speedy_fwd_vjp")

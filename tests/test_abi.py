@@ -82,3 +82,26 @@ def test_argument_errors(kw, code):
     rc = _call(**kw)
     assert rc == code
     assert _native.load().speedy_last_error().decode()
+
+
+def test_abi_struct_caches_follow_changes():
+    """The per-call host path rebuilds the parameter and vertical ABI structs only
+    when a value changed: an in-place edit of a parameter group is seen, an equal
+    copy reuses the struct, and the vertical struct follows the geometry's levels."""
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    a = phys._params()
+    assert phys._params() is a
+    phys.parameters.convection.rhbl = 0.81
+    b = phys._params()
+    assert b is not a and abs(b.rhbl - 0.81) < 1e-6
+    phys.parameters = Parameters.default().replace(convection=dict(rhbl=0.81))
+    assert phys._params() is b
+    g8, g7 = Geometry.from_grid_shape((64, 32), 8), Geometry.from_grid_shape((64, 32), 7)
+    v8 = phys._vertical(g8)
+    assert phys._vertical(g8) is v8 and v8.kx == 8
+    assert phys._vertical(g7).kx == 7
+    g8.fsg = g8.fsg.copy()
+    g8.fsg[0] += 0.01
+    v8b = phys._vertical(g8)
+    assert v8b is not v8 and abs(v8b.fsg[0] - float(g8.fsg[0])) < 1e-7
