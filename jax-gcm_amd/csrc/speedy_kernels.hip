@@ -765,15 +765,20 @@ __global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) { member_
 constexpr uint32_t kSegShift = 20, kSegLocalMask = (1u << kSegShift) - 1u;
 
 template <int KX>
-__global__ __launch_bounds__(kBlock) void levelset_batch_kernel(const KArgs* segs, const uint32_t* table) {
+__global__ __launch_bounds__(kBlock) void levelset_batch_kernel(const KArgs* __restrict__ segs,
+                                                                const uint32_t* __restrict__ table) {
   extern __shared__ uint32_t s_or[];
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
   levelset_body<KX>(segs[e >> kSegShift], e & kSegLocalMask, s_or);
 }
 
 // Forward-only segments of a batch: the forward without the adjoint's registers.
+// The descriptors are read-only and alias nothing the kernel writes
+// (__restrict__), so their fields are scalar loads and the field pointers read
+// from them are global-memory pointers.
 template <int KX>
-__global__ __launch_bounds__(kBlock) void fwd_only_batch_kernel(const KArgs* segs, const uint32_t* table) {
+__global__ __launch_bounds__(kBlock) void fwd_only_batch_kernel(const KArgs* __restrict__ segs,
+                                                                const uint32_t* __restrict__ table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
   fwd_body<KX, false, false, false, true>(segs[e >> kSegShift], e & kSegLocalMask);
 }
@@ -784,7 +789,8 @@ __global__ __launch_bounds__(kBlock) void fwd_batch_kernel(const KArgs* segs, co
   fwd_body<KX, false, false, true, true>(segs[e >> kSegShift], e & kSegLocalMask);
 }
 
-__global__ __launch_bounds__(kBlock) void param_reduce_batch_kernel(const KArgs* segs, const uint32_t* table) {
+__global__ __launch_bounds__(kBlock) void param_reduce_batch_kernel(const KArgs* __restrict__ segs,
+                                                                    const uint32_t* __restrict__ table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
   member_reduce_body(segs[e >> kSegShift], e & kSegLocalMask);
 }
