@@ -19,7 +19,7 @@ import torch
 
 from . import _native
 from .date import DateData
-from .forcing import FORCING_FIELDS, ForcingData, day_slice
+from .forcing import FORCING_FIELDS, ForcingData
 from .geometry import Geometry
 from .params import VJP_PARAMS, Parameters
 from .physics_interface import Physics, PhysicsState, PhysicsTendency
@@ -35,6 +35,17 @@ def _device(x, dev) -> torch.Tensor:
             return x
         return x.to(device=dev, dtype=torch.float32).contiguous()
     return torch.as_tensor(np.ascontiguousarray(x, np.float32), device=dev)
+
+
+def climatology_rows(arr, dev) -> torch.Tensor:
+    """A forcing climatology (ix, il, ndays) laid out as [ndays][ix * il] on `dev`:
+    row d is arr[:, :, d] in the kernels' column order (c = i * il + j), so the day
+    slice of a call (speedy_physics.py:119-121) is one contiguous row."""
+    t = arr if isinstance(arr, torch.Tensor) else torch.as_tensor(np.asarray(arr, np.float32))
+    if t.dim() != 3:
+        raise ValueError(f"a climatology is (ix, il, ndays), got shape {tuple(t.shape)}")
+    ix, il, nd = t.shape
+    return t.to(device=dev, dtype=torch.float32).permute(2, 0, 1).contiguous().view(nd, ix * il)
 
 
 class PhysicsData:
@@ -103,6 +114,7 @@ class SpeedyPhysics(Physics):
         self._abi_params_src = None
         self._verts: dict = {}
         self._fg: dict = {}
+        self._climates: dict = {}
         self.last_budgets = None
 
     # ------------------------------------------------------------------ helpers
@@ -143,6 +155,20 @@ class SpeedyPhysics(Physics):
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _climatology(self, arr) -> torch.Tensor:
+        """climatology_rows(arr) on the device, made once per array (per version of a
+        tensor, so in-place updates are seen); the calls then read the day's row."""
+        key = id(arr)
+        ver = arr._version if isinstance(arr, torch.Tensor) else None
+        hit = self._climates.get(key)
+        if hit is not None and hit[0] is arr and hit[1] == ver:
+            return hit[2]
+        rows = climatology_rows(arr, self.device)
+        if len(self._climates) > 16:
+            self._climates.clear()
+        self._climates[key] = (arr, ver, rows)
+        return rows
 
     def _const(self, arr) -> torch.Tensor:
         """Device copy of a geometry/forcing array, cached while the array lives."""
@@ -186,8 +212,9 @@ class SpeedyPhysics(Physics):
     def _forcing_fields(self, forcing: ForcingData, geometry: Geometry, day: int, ix: int, il: int):
         """(field, device tensor) of the forcing at `day` and the geometry.  Cached for
         the same forcing and geometry objects and day when every array is used by
-        identity (NumPy arrays through _const's cache, device tensors as they are);
-        day slices, broadcasts and host tensors are copied again on every call."""
+        identity (NumPy arrays through _const's and _climatology's caches, device
+        tensors as they are); broadcasts, host tensors and tensor climatologies (whose
+        version is checked) are looked up again on every call."""
         key = (id(forcing), id(geometry), day, ix, il)
         hit = self._fg.get(key)
         if hit is not None and hit[0] is forcing and hit[1] is geometry:
@@ -198,13 +225,17 @@ class SpeedyPhysics(Physics):
         srcs += [(src, getattr(geometry, src)) for src in ("fmask", "orog", "phis0", "sia", "coa")]
         for i, (dst, src) in enumerate(srcs):
             arr = src
-            if i < len(fmap):
-                if isinstance(arr, np.ndarray):
-                    arr = day_slice(arr, day)
-                    if arr.shape != (ix, il):
-                        arr = np.broadcast_to(arr, (ix, il))
-                elif arr.dim() > 2:
-                    arr = arr[:, :, min(max(day, 0), arr.shape[2] - 1)]
+            if i < len(fmap) and arr.ndim > 2:
+                # a climatology: the day's row of its [day][ncol] layout (made once);
+                # the day is clamped like the reference's gather
+                if tuple(arr.shape[:2]) != (ix, il):
+                    raise ValueError(f"forcing {dst} has shape {tuple(arr.shape)}, expected ({ix}, {il}, ndays)")
+                rows = self._climatology(arr)
+                by_identity &= not isinstance(arr, torch.Tensor)
+                out.append((dst, rows[min(max(day, 0), rows.shape[0] - 1)]))
+                continue
+            if i < len(fmap) and isinstance(arr, np.ndarray) and arr.shape != (ix, il):
+                arr = np.broadcast_to(arr, (ix, il))
             x = self._const(arr)
             by_identity &= arr is src and (isinstance(arr, np.ndarray) or x is arr)
             out.append((dst, x))

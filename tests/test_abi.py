@@ -105,3 +105,17 @@ def test_abi_struct_caches_follow_changes():
     g8.fsg[0] += 0.01
     v8b = phys._vertical(g8)
     assert v8b is not v8 and abs(v8b.fsg[0] - float(g8.fsg[0])) < 1e-7
+
+
+def test_climatology_rows_layout():
+    """A forcing climatology (ix, il, ndays) is laid out as [ndays][ix * il]: row d is
+    the day-d slice in the kernels' column order (c = i * il + j)."""
+    import numpy as np
+    import torch
+    from jcm_amd.speedy_physics import climatology_rows
+    arr = np.random.default_rng(0).standard_normal((6, 4, 9)).astype(np.float32)
+    for src in (arr, torch.as_tensor(arr)):
+        rows = climatology_rows(src, torch.device("cpu"))
+        assert tuple(rows.shape) == (9, 24) and rows.is_contiguous()
+        for d in range(9):
+            np.testing.assert_array_equal(rows[d].numpy(), arr[:, :, d].reshape(-1))

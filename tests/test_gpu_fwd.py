@@ -273,6 +273,55 @@ def test_fwd_repeated_calls_reset_level_set(phys):
         assert S == S_ref, (bin(S), bin(S_ref))
 
 
+def test_fwd_forcing_climatology_day_rows():
+    """Forcing climatologies (ix, il, 365), as NumPy arrays and as device tensors, give
+    exactly the tendencies of their day slice (the library lays each out once as
+    [day][ncol] and passes the day's row); the day is clamped like the reference's
+    gather (day 365 -> 364); the oracle, slicing the same climatologies, agrees; and
+    an in-place update of a device climatology is seen by the next call."""
+    import torch
+    from jcm_amd.forcing import FORCING_FIELDS
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    st, fo, ge, da = synthetic.make_case(64, 32, 8, seed=2, land=True)
+    days = np.arange(365, dtype=np.float32)
+    wave = lambda amp, ph: (1.0 + amp * np.sin(2 * np.pi * (days + ph) / 365.0)).astype(np.float32)
+    clim = {f: (np.asarray(getattr(fo, f), np.float32)[..., None] * wave(0.02, 30 * i)).astype(np.float32)
+            for i, f in enumerate(FORCING_FIELDS)}
+    fo3 = fo.copy(**clim)
+    dev = phys.device
+    fo3_dev = fo.copy(**{f: torch.as_tensor(v, device=dev) for f, v in clim.items()})
+    for tyear, day in ((0.0, 0), (100.4 / 365.2425, 100), (1.0, 364)):
+        date = da.copy(tyear=tyear)
+        assert min(date.model_day(), 364) == day
+        fo2 = fo.copy(**{f: np.ascontiguousarray(v[:, :, day]) for f, v in clim.items()})
+        t2, _ = phys.compute_tendencies(st, fo2, ge, date)
+        t3, _ = phys.compute_tendencies(st, fo3, ge, date)
+        t3d, _ = phys.compute_tendencies(st, fo3_dev, ge, date)
+        torch.cuda.synchronize()
+        for f in FIELDS:
+            ref2 = _np(getattr(t2, f))
+            np.testing.assert_array_equal(_np(getattr(t3, f)), ref2, err_msg=f"day {day} {f} numpy")
+            np.testing.assert_array_equal(_np(getattr(t3d, f)), ref2, err_msg=f"day {day} {f} device")
+        if day == 100:
+            ref, _, _ = H.run_oracle(st, fo3, ge, date)
+            ref64, _, _ = H.run_oracle(st, fo3, ge, date, dtype=np.float64)
+            for f in FIELDS:
+                ok, e32, e64, eref = H.parity(_np(getattr(t3, f)), getattr(ref, f), getattr(ref64, f))
+                print(f"climatology day 100 {f}: vs fp32 oracle {e32:.2e}, vs fp64 {e64:.2e}", flush=True)
+                assert ok, f"{f}: {e32:.3e}"
+    # an in-place update of a device climatology reaches the next call
+    date = da.copy(tyear=100.4 / 365.2425)
+    fo3_dev.sea_surface_temperature[:, :, 100] += 1.0
+    t3d, _ = phys.compute_tendencies(st, fo3_dev, ge, date)
+    fo2 = fo.copy(**{f: np.ascontiguousarray(v[:, :, 100]) for f, v in clim.items()})
+    fo2 = fo2.copy(sea_surface_temperature=fo2.sea_surface_temperature + np.float32(1.0))
+    t2, _ = phys.compute_tendencies(st, fo2, ge, date)
+    torch.cuda.synchronize()
+    for f in FIELDS:
+        np.testing.assert_array_equal(_np(getattr(t3d, f)), _np(getattr(t2, f)), err_msg=f"updated {f}")
+
+
 def test_fwd_convection_fortran_fixture(phys):
     """The reference's Fortran convection fixture (convection_test.py:109-140) through the
     full kernel with only the convection tendency summed."""
