@@ -230,6 +230,13 @@ int speedy_fwd_batch(const SpeedySegment *segs, int32_t nseg, const SpeedyVertic
 int speedy_read_level_sets(const void *workspace, size_t workspace_bytes, int32_t nmember,
                            uint32_t *out, void *stream);
 
+/* Running mean of the diagnostics on the device: acc[e] += diag[e] /
+ * steps_to_average for e < n, in float32 with an IEEE division, on `stream`.
+ * Replaces the tree_map add of DiagnosticsCollector.accumulate_if_physical_step
+ * (jcm/model.py:57-64) for the PhysicsData subset the forward writes (diag of
+ * SpeedyFields); acc is one output slot of the stacked averages. */
+int speedy_diag_accumulate(const float *diag, float *acc, int64_t n, float steps_to_average, void *stream);
+
 /* Kernel timers (no counterpart in the reference; for measurement).  While
  * enabled, every call records HIP events on its stream before and after each
  * of its kernels (up to `capacity` calls, then recording stops).

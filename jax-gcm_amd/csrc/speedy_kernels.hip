@@ -809,6 +809,29 @@ __global__ __launch_bounds__(kBlock) void seg_desc_kernel(KArgs a, KArgs* dst, u
   if (i < nblk + nls + nred) tbl[i] = i < nblk ? i : (i < nblk + nls ? i - nblk : i - nblk - nls);
 }
 
+// acc[e] += diag[e] / steps for e < n (the diagnostics' running mean): four
+// elements per thread (16-byte loads and stores when both buffers allow), a
+// grid-stride loop, and the scalar tail by the first threads.
+__global__ __launch_bounds__(kBlock) void diag_accumulate_kernel(const float* __restrict__ d, float* __restrict__ acc,
+                                                                 int64_t n, float steps, int vec) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const int64_t t0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t done = 0;
+  if (vec) {
+    const int64_t n4 = n / 4;
+    const float4* d4 = reinterpret_cast<const float4*>(d);
+    float4* a4 = reinterpret_cast<float4*>(acc);
+    for (int64_t i = t0; i < n4; i += stride) {
+      const float4 x = d4[i];
+      float4 y = a4[i];
+      y.x += x.x / steps; y.y += x.y / steps; y.z += x.z / steps; y.w += x.w / steps;
+      a4[i] = y;
+    }
+    done = n4 * 4;
+  }
+  for (int64_t i = done + t0; i < n; i += stride) acc[i] += d[i] / steps;
+}
+
 __global__ void fill_zero_kernel(uint32_t* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1320,6 +1343,20 @@ int speedy_read_level_sets(const void* workspace, size_t workspace_bytes, int32_
                                 hipMemcpyDeviceToHost, (hipStream_t)stream);
   if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: copy failed: ") + hipGetErrorString(e));
+  return SPEEDY_OK;
+}
+
+int speedy_diag_accumulate(const float* diag, float* acc, int64_t n, float steps_to_average, void* stream) {
+  if (n < 0 || !(steps_to_average > 0.0f)) return fail(SPEEDY_ERR_ARG, "speedy_diag_accumulate: bad n or steps");
+  if (n == 0) return SPEEDY_OK;
+  if (!diag || !acc) return fail(SPEEDY_ERR_ARG, "speedy_diag_accumulate: NULL buffer");
+  const int vec = ((reinterpret_cast<uintptr_t>(diag) | reinterpret_cast<uintptr_t>(acc)) % 16u) == 0u;
+  const int64_t work = vec ? (n + 3) / 4 : n;
+  const int64_t blocks = std::min<int64_t>((work + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(diag_accumulate_kernel, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, diag, acc,
+                     n, steps_to_average, vec);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_diag_accumulate: ") + hipGetErrorString(e));
   return SPEEDY_OK;
 }
 

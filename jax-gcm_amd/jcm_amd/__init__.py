@@ -2,6 +2,7 @@
 
 Public API mirrors the reference modules it replaces:
   SpeedyPhysics            jcm/physics/speedy/speedy_physics.py
+  DiagnosticsCollector     jcm/model.py (the diagnostics' running mean, on the device)
   Parameters               jcm/physics/speedy/params.py
   PhysicsState, PhysicsTendency, Physics, verify_state, verify_tendencies
                            jcm/physics_interface.py
@@ -16,11 +17,11 @@ from .params import Parameters
 from .physics_interface import Physics, PhysicsState, PhysicsTendency, verify_state, verify_tendencies
 
 __all__ = ["DateData", "ForcingData", "Geometry", "Parameters", "Physics", "PhysicsState", "PhysicsTendency",
-           "verify_state", "verify_tendencies", "SpeedyPhysics"]
+           "verify_state", "verify_tendencies", "SpeedyPhysics", "DiagnosticsCollector"]
 
 
 def __getattr__(name):
-    if name == "SpeedyPhysics":
-        from .speedy_physics import SpeedyPhysics
-        return SpeedyPhysics
+    if name in ("SpeedyPhysics", "DiagnosticsCollector"):
+        from . import speedy_physics
+        return getattr(speedy_physics, name)
     raise AttributeError(name)

@@ -127,6 +127,8 @@ def load(path: str | None = None):
     lib.speedy_read_level_sets.restype = ctypes.c_int
     lib.speedy_read_level_sets.argtypes = [CFP, ctypes.c_size_t, ctypes.c_int32,
                                            ctypes.POINTER(ctypes.c_uint32), CFP]
+    lib.speedy_diag_accumulate.restype = ctypes.c_int
+    lib.speedy_diag_accumulate.argtypes = [CFP, CFP, ctypes.c_int64, ctypes.c_float, CFP]
     lib.speedy_timer_enable.restype = ctypes.c_int
     lib.speedy_timer_enable.argtypes = [ctypes.c_int32]
     lib.speedy_timer_disable.restype = ctypes.c_int

@@ -98,6 +98,41 @@ class PhysicsData:
         return SimpleNamespace(**ns)
 
 
+class DiagnosticsCollector:
+    """DiagnosticsCollector (model.py:46-64) for the PhysicsData subset this library
+    writes, with the averages on the device.  ``data`` holds the stacked averages of
+    the outputs (init_data stacks an empty data struct, as model.py:88-89 does);
+    accumulate_if_physical_step adds new_data / steps_to_average into output ``i``
+    once per physics step (a HIP kernel, speedy_diag_accumulate, in the reference's
+    float32 arithmetic) and clears ``physical_step``, which the caller sets again
+    before each step (model.py:103)."""
+
+    def __init__(self, steps_to_average: int):
+        self.i = 0
+        self.physical_step = True
+        self.steps_to_average = steps_to_average
+        self.data = None
+
+    def init_data(self, empty_data: PhysicsData, outer_steps: int) -> None:
+        d = empty_data._diag
+        stacked = d.unsqueeze(0).repeat((outer_steps,) + (1,) * d.dim()).contiguous()
+        self.data = PhysicsData(stacked, empty_data._names, empty_data._kx, empty_data._hshape)
+
+    def accumulate_if_physical_step(self, new_data: PhysicsData) -> None:
+        if not self.physical_step:
+            return
+        if self.data is None:
+            raise _native.SpeedyError("DiagnosticsCollector: init_data first")
+        src, dst = new_data._diag, self.data._diag[self.i]
+        if tuple(src.shape) != tuple(dst.shape) or src.device != dst.device or not src.is_contiguous():
+            raise ValueError(f"diagnostics of shape {tuple(src.shape)} do not match the averages {tuple(dst.shape)}")
+        stream = ctypes.c_void_p(torch.cuda.current_stream(src.device).cuda_stream)
+        _native.check(_native.load().speedy_diag_accumulate(src.data_ptr(), dst.data_ptr(), src.numel(),
+                                                            float(self.steps_to_average), stream),
+                      "speedy_diag_accumulate")
+        self.physical_step = False
+
+
 class SpeedyPhysics(Physics):
     """SPEEDY physics (speedy_physics.py:34-133) fused into gfx950 kernels."""
 

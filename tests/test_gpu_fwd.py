@@ -473,3 +473,52 @@ def test_fwd_float64_budgets(phys, nlon, nlat, verify):
     if not verify:   # the column water budget closes (verify_tendencies' limiter breaks it, as in the reference)
         assert abs(b[4]) <= 1e-5 * wscale and abs(ref[4]) <= 1e-6 * wscale
     assert b[5] >= 0.0
+
+
+def test_diagnostics_collector_running_mean():
+    """DiagnosticsCollector (model.py:46-64) on the device: over 2 outputs of 3 physics
+    steps each, with a second accumulate per step that must be a no-op, the averages
+    equal the reference's float32 sequence data[i] += new / steps bit for bit."""
+    import torch
+    from jcm_amd import DiagnosticsCollector
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics(with_diagnostics=True)
+    st, fo, ge, da = synthetic.make_case(64, 32, 8, seed=7, land=True)
+    col = DiagnosticsCollector(steps_to_average=3)
+    col.init_data(phys.get_empty_data(ge), outer_steps=2)
+    ref = np.zeros(tuple(col.data._diag.shape), np.float32)
+    for o in range(2):
+        for k in range(3):
+            case = wet((st, fo, ge, da.copy(model_step=k)), 10 * o + k)
+            col.physical_step = True
+            _, data = phys.compute_tendencies(*case)
+            col.accumulate_if_physical_step(data)
+            col.accumulate_if_physical_step(data)   # physical_step is now False: no-op
+            torch.cuda.synchronize()
+            ref[o] = ref[o] + _np(data._diag) / np.float32(3)
+        col.i += 1
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(_np(col.data._diag), ref)
+    rsds = col.data.shortwave_rad.rsds            # the struct views work on the stacked averages
+    assert tuple(rsds.shape) == (2, 64, 32)
+
+
+@pytest.mark.parametrize("n,offset", [(1, 0), (4099, 0), (4099, 1), (1 << 20, 3)])
+def test_diag_accumulate_tails_and_alignment(n, offset):
+    """speedy_diag_accumulate on lengths that are not a multiple of 4 and on buffers
+    that are not 16-byte aligned (scalar path) against the float32 sequence."""
+    import ctypes
+    import torch
+    from jcm_amd import _native
+    lib = _native.load()
+    rng = np.random.default_rng(n + offset)
+    d = rng.standard_normal(n + 8).astype(np.float32)
+    a = rng.standard_normal(n + 8).astype(np.float32)
+    dd, ad = torch.as_tensor(d, device="cuda"), torch.as_tensor(a, device="cuda")
+    for _ in range(2):
+        rc = lib.speedy_diag_accumulate(dd[offset:].data_ptr(), ad[offset:].data_ptr(), n, 7.0,
+                                        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        assert rc == 0, _native.load().speedy_last_error()
+        a[offset:offset + n] = a[offset:offset + n] + d[offset:offset + n] / np.float32(7.0)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(_np(ad), a)
