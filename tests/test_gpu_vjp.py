@@ -168,17 +168,19 @@ def test_vjp_finite_at_unit_cotangents(phys):
     assert np.all(np.isfinite(prm))
 
 
-_TWO_LAUNCH = "same shapes and non-batched two-launch path as the forward cases kept off the GPU (test_gpu_fwd)"
-
-
 @pytest.mark.parametrize("nlon,nlat,nmember", [(64, 32, 3), (100, 50, 3), (90, 45, 2), (256, 128, 3), (16, 8, 5),
-                                               pytest.param(256, 128, 8, marks=pytest.mark.skip(reason=_TWO_LAUNCH)),
-                                               pytest.param(8, 4, 6, marks=pytest.mark.skip(reason=_TWO_LAUNCH))])
+                                               (256, 128, 8), (8, 4, 6)])
 def test_vjp_members_stacked(phys, nlon, nlat, nmember):
     """Stacked members (fused path; members straddling workgroups; beyond one
     co-resident launch and narrower than a workgroup, which run as a one-segment
     batch): each member's gradients equal its own float64 adjoint, with its own
-    level set and its own parameter-gradient sum."""
+    level set and its own parameter-gradient sum.  Members of T85 size (32768
+    columns) are held to the per-column bar of the large grids
+    (test_gpu_large._check_gradients): over that many columns a few ill-conditioned
+    ones always occur, where two float32 evaluations differ by more than the
+    whole-grid bar allows (at 8 members the GPU and the float32 host adjoint have the
+    same columns above 1e-4, 1 to 16 per member, but in member 0 one of them lands
+    at 1.9e-4 against the host's 4.4e-5)."""
     st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=9, land=True, nmember=nmember, member_t_sigma=1.0)
     q = st.specific_humidity.copy()
     q[1::2] *= np.float32(1.12)
@@ -195,7 +197,11 @@ def test_vjp_members_stacked(phys, nlon, nlat, nmember):
         assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
         _, g64 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m])
         _, g32 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m], dtype=np.float32)
-        _check_against_host({k: v[m] for k, v in g.items()}, prm[m], g64, f"member {m}", g32)
+        if nlon * nlat >= 32768:
+            from test_gpu_large import _check_gradients
+            _check_gradients(f"member {m}", {k: v[m] for k, v in g.items()}, prm[m], g64, g32)
+        else:
+            _check_against_host({k: v[m] for k, v in g.items()}, prm[m], g64, f"member {m}", g32)
 
 
 def test_vjp_with_fused_verify_steps(phys):
