@@ -310,8 +310,8 @@ __device__ __forceinline__ uint32_t chunk_word(const KArgs& a, uint32_t b, uint3
 // Two-kernel path, member wider than a block: this thread's share of the OR over
 // the chunks covering member m, kChunkLoads words per thread.  Indexes past the
 // member's last chunk are clamped to it (OR is idempotent), so the loads carry
-// no branches and issue together.  A member spans at most ls_blocks <= 1024 =
-// kChunkLoads * kBlock chunks (make_grid).
+// no branches and issue together.  A member spans at most ls_blocks <= 512
+// chunks (make_grid), within the kChunkLoads * kBlock = 1024 this reaches.
 constexpr uint32_t kChunkLoads = 4;
 __device__ __forceinline__ uint32_t chunk_share(const KArgs& a, uint32_t m) {
   const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
@@ -883,7 +883,11 @@ Grid make_grid(int64_t nlon, int64_t nlat, int64_t nmember) {
   g.ncol = (uint32_t)(nlon * nlat);
   g.ntot = (uint32_t)(nlon * nlat * nmember);
   g.nblk = (g.ntot + kBlock - 1) / kBlock;
-  const uint32_t lsb = g.nblk < 1024u ? g.nblk : 1024u;   // ~4 blocks per CU: latency hiding
+  // At most 512 level-set chunks (~2 blocks per CU): the forward + VJP kernels
+  // gather a member's chunk words in one wave, one round trip per 64 chunks, and at
+  // T425 512 chunks cost that gather 20 us less than 1024 (fwd+VJP 892 vs 912 us)
+  // while the level-set pass itself stays at 25 us (256 chunks: 33 us).
+  const uint32_t lsb = g.nblk < 512u ? g.nblk : 512u;
   g.chunk = ((g.ntot + lsb - 1) / lsb + kBlock - 1) / kBlock * kBlock;
   g.ls_blocks = (g.ntot + g.chunk - 1) / g.chunk;
   g.jmax = (g.chunk - 1) / g.ncol + 2;
@@ -946,7 +950,7 @@ int prepare(const SpeedyFields* fields, const SpeedyVertical* vert, const Speedy
   if (ntot >= ((int64_t)1 << 31) - kBlock) return fail(SPEEDY_ERR_CAPACITY, "speedy: more than 2^31 columns per call");
   const Grid gr = make_grid(nlon, nlat, nmember);
   if (gr.jmax > kMaxJ) return fail(SPEEDY_ERR_CAPACITY, "speedy: too many members per column chunk");
-  if (gr.ls_blocks > kChunkLoads * kBlock)   // chunk_share's reach (make_grid keeps it <= 1024)
+  if (gr.ls_blocks > kChunkLoads * kBlock)   // chunk_share's reach (make_grid keeps it <= 512)
     return fail(SPEEDY_ERR_CAPACITY, "speedy: too many level-set chunks");
   const WsLayout L = ws_layout(gr, nmember);
   if (!workspace) return fail(SPEEDY_ERR_WORKSPACE, "speedy: workspace is NULL");
