@@ -5,6 +5,8 @@
   same request called alone: tendencies, state gradients and parameter
   gradients equal within 1e-6 relative (measured: identical bits), and the same
   level set.
+* A state written on the client's stream just before the request, while that
+  stream is still busy, is the state the batch reads.
 * Admission respects the HBM budget: with a budget that holds only a few
   requests, the loop runs several batches, never holds more than the budget,
   and admits requests that arrive while earlier batches are on the GPU.
@@ -122,6 +124,44 @@ def test_batched_ragged_members_equal_alone():
                 _same(getattr(r.state_grads, f), getattr(sg, f), f"req {r.rid} grad {f}")
             for k in pg:
                 _same(r.param_grads[k], pg[k], f"req {r.rid} {k}")
+
+
+@pytest.mark.parametrize("batching", [True, False])
+def test_batch_waits_for_client_stream_writes(batching):
+    """A state written on the client's stream just before the request (behind
+    about ten milliseconds of other work on that stream) is what the server reads:
+    the batch is ordered after the client's stream when that stream is busy."""
+    import torch
+    from jcm_amd.serving import PhysicsServer, Request
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2))
+    st, fo, ge, da = _device_case(256, 128, 77)
+    shp = st.temperature.shape
+    cot = PhysicsTendency(*[torch.randn(shp, device=st.temperature.device) for _ in range(4)])
+    new_t = st.temperature + 3.0 * torch.sin(torch.arange(st.temperature.numel(), device=st.temperature.device)
+                                              .view(shp).float() * 0.37)
+    old_t = st.temperature.clone()
+    st.temperature.copy_(new_t)
+    want_f, _ = phys.compute_tendencies(st, fo, ge, da)
+    want_v, want_g, _ = phys.compute_tendencies_vjp(st, fo, ge, da, cot)
+    want_f = want_f.temperature.clone()
+    want_v, want_g = want_v.temperature.clone(), want_g.temperature.clone()
+    st.temperature.copy_(old_t)
+    a = torch.randn(4096, 4096, device=st.temperature.device)
+    torch.cuda.synchronize()
+    for _ in range(8):
+        a = torch.mm(a, a) * 1e-3
+    st.temperature.copy_(new_t)
+    busy = not torch.cuda.current_stream().query()
+    reqs = [Request(state=st, forcing=fo, geometry=ge, date=da, rid=0),
+            Request(state=st, forcing=fo, geometry=ge, date=da, cotangent=cot, rid=1)]
+    server = PhysicsServer(phys, n_streams=2, batching=batching)
+    server.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    assert busy, "the client stream finished before the request was served; the wait was not exercised"
+    _same(reqs[0].tendency.temperature, want_f, "forward dT")
+    _same(reqs[1].tendency.temperature, want_v, "forward + VJP dT")
+    _same(reqs[1].state_grads.temperature, want_g, "dL/dT")
 
 
 def test_admission_respects_hbm_budget_and_overlaps():
