@@ -28,6 +28,13 @@ STATE_FIELDS = ("u_wind", "v_wind", "temperature", "specific_humidity", "geopote
 VERIFY_FLAGS = _native.FLAG_VERIFY_STATE | _native.FLAG_VERIFY_TENDENCIES
 
 
+# torch's raw current-stream query (the one its compiled kernels use): about 0.2 us,
+# against about 2 us for torch.cuda.current_stream(device).cuda_stream
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_STATE_FIELDS = (("u_wind", "u"), ("v_wind", "v"), ("temperature", "t"), ("specific_humidity", "q"),
+                 ("geopotential", "phi"))
+
+
 def _device(x, dev) -> torch.Tensor:
     """float32 contiguous tensor on `dev`; tensors already there are used as is."""
     if isinstance(x, torch.Tensor):
@@ -151,11 +158,15 @@ class SpeedyPhysics(Physics):
         self._fg: dict = {}
         self._climates: dict = {}
         self.last_budgets = None
+        self._gpu_ok = False
 
     # ------------------------------------------------------------------ helpers
     def _require_gpu(self):
+        if self._gpu_ok:
+            return
         if self.device is None or self.device.type != "cuda" or not torch.cuda.is_available():
             raise _native.SpeedyError("SpeedyPhysics needs a ROCm GPU; there is no CPU path")
+        self._gpu_ok = True
 
     def _params(self) -> _native.SpeedyParams:
         """parameters.to_abi(), rebuilt only when a value changed (also in place)."""
@@ -189,6 +200,9 @@ class SpeedyPhysics(Physics):
         return ws
 
     def _stream(self):
+        if _RAW_STREAM is not None:
+            idx = self.device.index
+            return ctypes.c_void_p(_RAW_STREAM(torch.cuda.current_device() if idx is None else idx))
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _climatology(self, arr) -> torch.Tensor:
@@ -227,8 +241,7 @@ class SpeedyPhysics(Physics):
             raise ValueError(f"state shape {tuple(t.shape)} does not match geometry {geometry.nodal_shape}")
         nmember = 1 if t.dim() == 3 else int(np.prod(t.shape[:-3]))
         f = _native.SpeedyFields()
-        names = {"u_wind": "u", "v_wind": "v", "temperature": "t", "specific_humidity": "q", "geopotential": "phi"}
-        for src, dst in names.items():
+        for src, dst in _STATE_FIELDS:
             x = _device(getattr(state, src), self.device)
             if x.shape != t.shape:
                 raise ValueError(f"{src} has shape {tuple(x.shape)}, expected {tuple(t.shape)}")
