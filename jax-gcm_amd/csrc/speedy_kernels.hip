@@ -20,9 +20,8 @@
 //     every batch): a level-set kernel, a fixed number of blocks each covering a
 //     contiguous chunk of columns, writes per-(chunk, member) ORs; the forward
 //     kernel ORs the few chunks of its member, recomputes phase A and does
-//     phase B.  Single calls without diagnostics run this as a one-segment batch
-//     (seg_desc_kernel writes the descriptor into the workspace); calls with
-//     diagnostics use levelset_kernel and fwd_kernel<..., FUSED = false, ...>.
+//     phase B.  Single calls run this as a one-segment batch (seg_desc_kernel
+//     writes the descriptor into the workspace).
 //
 // One thread per column; the column lives in registers.  Accesses are
 // [field][level][column] with consecutive lanes on consecutive columns
@@ -292,8 +291,8 @@ __device__ __forceinline__ uint32_t collect_fused_t(const KArgs& a, uint32_t tag
 // The forward + VJP kernels' form (an outlined call, shuffle OR).
 __device__ uint32_t collect_fused(const KArgs& a, uint32_t tag, uint32_t m) { return collect_fused_t<false>(a, tag, m); }
 
-// Two-kernel path: OR over the chunks of levelset_kernel that cover member m, by
-// one wave (the forward + VJP kernels' form).
+// Two-kernel path: OR over the level-set chunks that cover member m, by one wave
+// (the forward + VJP kernels' form).
 __device__ uint32_t collect_chunks(const KArgs& a, uint32_t m) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
@@ -302,7 +301,7 @@ __device__ uint32_t collect_chunks(const KArgs& a, uint32_t m) {
   return wave_or<false>(acc);
 }
 
-// Two-kernel path: member m's word of level-set chunk b (levelset_kernel output).
+// Two-kernel path: member m's word of level-set chunk b (levelset_batch_kernel output).
 __device__ __forceinline__ uint32_t chunk_word(const KArgs& a, uint32_t b, uint32_t m) {
   return a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
 }
@@ -349,12 +348,6 @@ __device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[bid * a.jmax + i] = s_or[i];
-}
-
-template <int KX>
-__global__ __launch_bounds__(kBlock) void levelset_kernel(KArgs a) {
-  extern __shared__ uint32_t s_or[];   // jmax words
-  levelset_body<KX>(a, blockIdx.x, s_or);
 }
 
 // ---------------------------------------------------------------- VJP helpers
@@ -491,8 +484,8 @@ __device__ __forceinline__ void block_budget_partials(const KArgs& a, uint32_t b
 }
 
 // Forward (and, with VJP, the fused forward + VJP).  FUSED: single launch with
-// the split grid barrier (all blocks co-resident); otherwise S comes from
-// levelset_kernel of the same call.
+// the split grid barrier (all blocks co-resident); otherwise S comes from the
+// level-set kernel of the same batch.
 // EXTRA: the optional verify steps and float64 budgets are compiled in (their
 // flags and pointers are then read at run time); the plain forward omits them.
 template <int KX, bool DIAG, bool FUSED, bool VJP, bool EXTRA>
@@ -783,6 +776,15 @@ __global__ __launch_bounds__(kBlock) void fwd_only_batch_kernel(const KArgs* __r
   fwd_body<KX, false, false, false, true>(segs[e >> kSegShift], e & kSegLocalMask);
 }
 
+// Forward-only segments that also write the PhysicsData diagnostics: single
+// calls with diagnostics that the fused launch cannot take.
+template <int KX>
+__global__ __launch_bounds__(kBlock) void fwd_diag_batch_kernel(const KArgs* __restrict__ segs,
+                                                                const uint32_t* __restrict__ table) {
+  const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
+  fwd_body<KX, true, false, false, true>(segs[e >> kSegShift], e & kSegLocalMask);
+}
+
 template <int KX>
 __global__ __launch_bounds__(kBlock) void fwd_batch_kernel(const KArgs* segs, const uint32_t* table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
@@ -914,7 +916,7 @@ WsLayout ws_layout(const Grid& g, int32_t nmember) {
   return L;
 }
 
-constexpr uint32_t kMaxJ = 16384;   // LDS words of levelset_kernel's per-member ORs
+constexpr uint32_t kMaxJ = 16384;   // LDS words of the level-set kernel's per-member ORs
 
 Divisor make_divisor(uint32_t d) {
   Divisor dv;
@@ -1010,7 +1012,9 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
   static int64_t cap = -1;   // one GPU per process in this library's deployments
   if (cap < 0) {
     hipError_t se = ensure_stack(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
-    if (se == hipSuccess) se = VJP ? ensure_stack(fwd_batch_kernel<KX>) : ensure_stack(fwd_only_batch_kernel<KX>);
+    if (se == hipSuccess)
+      se = VJP ? ensure_stack(fwd_batch_kernel<KX>)
+               : DIAG ? ensure_stack(fwd_diag_batch_kernel<KX>) : ensure_stack(fwd_only_batch_kernel<KX>);
     if (se != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: stack limit: ") + hipGetErrorString(se));
     cap = coresident_blocks(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
   }
@@ -1033,7 +1037,7 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
       e = hipGetLastError();
     }
     timer_end(s, slot);
-  } else if (!DIAG) {
+  } else {
     // Grids the fused launch cannot take (beyond one co-resident launch, members
     // narrower than a workgroup, or a given level set) run as a one-segment batch
     // through the batch kernels; the workspace holds the descriptor and tables.
@@ -1058,6 +1062,9 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
     if (VJP)
       hipLaunchKernelGGL(fwd_batch_kernel<KX>, dim3(a.nblk), dim3(kBlock), 0, s, (const KArgs*)d_args,
                          (const uint32_t*)tbl);
+    else if (DIAG)
+      hipLaunchKernelGGL(fwd_diag_batch_kernel<KX>, dim3(a.nblk), dim3(kBlock), 0, s, (const KArgs*)d_args,
+                         (const uint32_t*)tbl);
     else
       hipLaunchKernelGGL(fwd_only_batch_kernel<KX>, dim3(a.nblk), dim3(kBlock), 0, s, (const KArgs*)d_args,
                          (const uint32_t*)tbl);
@@ -1071,20 +1078,6 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
     e = hipGetLastError();
     if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: launch failed: ") + hipGetErrorString(e));
     return SPEEDY_OK;
-  } else {
-    // with diagnostics: the non-batched two-launch kernels
-    int slot = -1;
-    if (a.level_set < 0) {
-      slot = timer_begin(s, 0);
-      hipLaunchKernelGGL(levelset_kernel<KX>, dim3(a.ls_blocks), dim3(kBlock), a.jmax * sizeof(uint32_t), s, a);
-      timer_end(s, slot);
-    }
-    // The two-launch forward always carries the optional verify / budget code
-    // (run-time flags), whatever EXTRA is: one kernel per (KX, DIAG, VJP).
-    slot = timer_begin(s, kind);
-    hipLaunchKernelGGL((fwd_kernel<KX, DIAG, false, VJP, true>), dim3(a.nblk), dim3(kBlock), 0, s, a);
-    timer_end(s, slot);
-    e = hipGetLastError();
   }
   if (e == hipSuccess && (VJP || a.f.budget)) {
     const int slot = timer_begin(s, 3);

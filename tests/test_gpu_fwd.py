@@ -259,6 +259,66 @@ def test_fwd_one_segment_batch_route(nlon, nlat, nmember, level_set):
                 assert abs(bud[m, i] - rb[i]) <= 1e-5 * wscale, (m, i, bud[m, i], rb[i])
 
 
+@pytest.mark.parametrize("nlon,nlat", [(256, 128), (64, 32)])
+def test_fwd_diagnostics_batch_route_equals_fused(nlon, nlat):
+    """A call with diagnostics and a given level set runs as a one-segment batch
+    through fwd_diag_batch_kernel.  Given the level set the fused launch computed
+    for the same state, its tendencies and PhysicsData subset equal the fused
+    call's (within 1e-6 relative; the same column code in both kernels)."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics(with_diagnostics=True)
+    case = wet(synthetic.make_case(nlon, nlat, 8, seed=19, land=True), 21)
+    t0, d0 = phys.compute_tendencies(*case)
+    torch.cuda.synchronize()
+    S0 = phys.level_sets(nlon, nlat, 1)[0]
+    t1, d1 = phys.compute_tendencies(*case, level_set=S0)
+    torch.cuda.synchronize()
+    identical = True
+    for f in FIELDS:
+        a, b = _np(getattr(t1, f)), _np(getattr(t0, f))
+        err = float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-30)
+        assert err <= 1e-6, f"{f}: {err:.2e}"
+        identical &= err == 0.0
+    a, b = _np(d1._diag), _np(d0._diag)
+    scale = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-30)
+    err = float((np.abs(a - b) / scale).max())
+    assert err <= 1e-6, f"diagnostics: {err:.2e}"
+    identical &= err == 0.0
+    print(f"{nlon}x{nlat}: batch route == fused bit for bit: {identical}", flush=True)
+
+
+def test_fwd_diagnostics_narrow_members():
+    """Members narrower than a workgroup (16 x 8 columns, 5 members) with
+    diagnostics: level-set launch, then fwd_diag_batch_kernel.  Per member: level
+    set, tendencies and the PhysicsData subset against the member's oracle run
+    (diagnostic fields bound as in test_fwd_diagnostics)."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics(with_diagnostics=True)
+    nlon, nlat, nmember = 16, 8, 5
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=23, land=True, nmember=nmember, member_t_sigma=0.5)
+    q = np.array(st.specific_humidity)
+    q[1::2] *= np.float32(1.15)
+    case = (st.copy(specific_humidity=q), fo, ge, da)
+    tend, data = phys.compute_tendencies(*case)
+    torch.cuda.synchronize()
+    S = phys.level_sets(nlon, nlat, nmember)
+    d = data.to_dict()
+    for m in range(nmember):
+        ref, rdata, S_ref = H.run_oracle(*case, member=m)
+        ref64, rdata64, _ = H.run_oracle(*case, member=m, dtype=np.float64)
+        assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
+        for f in FIELDS:
+            ok, e32, e64, eref = H.parity(_np(getattr(tend, f)[m]), getattr(ref, f), getattr(ref64, f))
+            assert ok, f"member {m} {f}: {e32:.3e} (fp64 {e64:.2e}, oracle {eref:.2e})"
+        checks, checks64 = _data_fields(rdata), _data_fields(rdata64)
+        for k, r in checks.items():
+            g = _np(d[k])[m]
+            e32, e64, eref = H.rel_inf(g, r), H.rel_inf(g, checks64[k]), H.rel_inf(r, checks64[k])
+            assert e32 <= 2e-5 or e64 <= max(2e-5, 2 * eref), f"member {m} {k}: {e32:.3e} (fp64 {e64:.3e})"
+
+
 def test_fwd_repeated_calls_reset_level_set(phys):
     """The level set of one call does not leak into the next (self-resetting workspace)."""
     import torch
@@ -396,8 +456,9 @@ def test_fwd_graph_replay_matches_eager(nlon, nlat, nmember):
 
 TWO_LAUNCH_SKIP = ("GPU memory faults were reported at the first synchronize of this test twice (T85 x 8 members, "
                    "then 8 x 4 x 6 members), each time right after the graph-replay test, with a plain-forward "
-                   "two-launch kernel that has since been taken out; a CPU simulation of every index of the path is "
-                   "in bounds.  No cause has been shown, so the test stays off the GPU.")
+                   "two-launch kernel that has since been taken out (the library no longer has the two-launch "
+                   "kernels; these grids now run as a one-segment batch); a CPU simulation of every index of the "
+                   "path is in bounds.  No cause has been shown, so the test stays off the GPU.")
 
 
 @pytest.mark.skip(reason=TWO_LAUNCH_SKIP)
