@@ -29,6 +29,7 @@ from __future__ import annotations
 import ctypes
 import math
 import time
+import weakref
 from collections import deque
 from dataclasses import dataclass, field
 from typing import Any, Callable, Optional
@@ -115,6 +116,7 @@ class PhysicsServer:
                  record_level_sets: bool = False):
         self.phys = physics or SpeedyPhysics()
         self.dev = self.phys.device
+        self._dev_index = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
         self.lib = self.phys._lib
         self.hbm_budget = int(hbm_budget)
         self.max_batch_columns = int(max_batch_columns)
@@ -133,7 +135,6 @@ class PhysicsServer:
         self._ws_bytes: dict = {}
         self._members: dict = {}
         self._verts: dict = {}
-        self._cots: dict = {}
 
     # ------------------------------------------------------------ resources
     def _workspace(self, key, stream) -> torch.Tensor:
@@ -178,15 +179,24 @@ class PhysicsServer:
         built once: members' states stay resident between requests."""
         key = (id(req.state), id(req.forcing), id(req.geometry), req.date.model_day())
         hit = self._members.get(key)
+        if hit is not None and not (hit[5][0]() is req.state and hit[5][1]() is req.forcing
+                                    and hit[5][2]() is req.geometry):
+            hit = None   # an id reused by a new object
         if hit is None:
             keep: list = []
             f, t, nm = self.phys._fields(req.state, req.forcing, req.geometry, req.date, keep)
             kx, ix, il = req.geometry.nodal_shape
             flags = _native.FLAG_LFLUXLAND if bool(req.forcing.lfluxland) else 0
-            hit = (f, tuple(t.shape), (ix, il, nm), flags, keep, (req.state, req.forcing, req.geometry))
-            if len(self._members) > 4096:
-                self._members.clear()
-            self._members[key] = hit
+            # weak references: the entry (and the device tensors it keeps) goes
+            # when the client drops the state, so states that are new with every
+            # request are not held here
+            members = self._members
+            drop = lambda _ref, k=key, d=members: d.pop(k, None)
+            refs = (weakref.ref(req.state, drop), weakref.ref(req.forcing), weakref.ref(req.geometry))
+            hit = (f, tuple(t.shape), (ix, il, nm), flags, keep, refs)
+            if len(members) > 4096:
+                members.clear()
+            members[key] = hit
         return hit
 
     def _prepare(self, req: Request, seg, stream):
@@ -220,21 +230,14 @@ class PhysicsServer:
         return (ix, il, nm), flags, buf
 
     def _cotangent_ptrs(self, ct: PhysicsTendency, tshape) -> tuple:
-        """Device addresses of a cotangent's four fields, checked once per
-        cotangent object (members' cotangents stay resident between requests)."""
+        """Device addresses of a cotangent's four fields, after checking them.
+        Cotangents are usually new with every request, so nothing is cached."""
         cts = (ct.u_wind, ct.v_wind, ct.temperature, ct.specific_humidity)
-        hit = self._cots.get(id(ct))
-        if hit is not None and hit[0] is ct and all(a is b for a, b in zip(hit[1], cts)) and hit[2] == tshape:
-            return hit[3]
         for c in cts:
-            if not (isinstance(c, torch.Tensor) and c.device == self.dev and c.dtype == torch.float32
-                    and c.is_contiguous() and tuple(c.shape) == tshape):
+            if not (isinstance(c, torch.Tensor) and c.dtype is torch.float32 and c.shape == tshape
+                    and c.get_device() == self._dev_index and c.is_contiguous()):
                 raise ValueError("cotangents must be contiguous float32 device tensors of the state's shape")
-        ptrs = tuple(c.data_ptr() for c in cts)
-        if len(self._cots) > 4096:
-            self._cots.clear()
-        self._cots[id(ct)] = (ct, cts, tshape, ptrs)
-        return ptrs
+        return tuple(c.data_ptr() for c in cts)
 
     # ------------------------------------------------------------ loop
     def submit(self, req: Request):

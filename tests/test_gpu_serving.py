@@ -164,6 +164,30 @@ def test_batch_waits_for_client_stream_writes(batching):
     _same(reqs[1].state_grads.temperature, want_g, "dL/dT")
 
 
+def test_member_cache_releases_dropped_states():
+    """States that are new with every request (a model stepping forward) are not
+    held by the server once the client drops them: the member cache keeps weak
+    references to the state, forcing and geometry."""
+    import gc
+    import torch
+    from jcm_amd.serving import PhysicsServer, Request
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2))
+    server = PhysicsServer(phys, n_streams=1)
+    st, fo, ge, da = _device_case(64, 32, 5)
+    fields = ("u_wind", "v_wind", "temperature", "specific_humidity", "geopotential", "normalized_surface_pressure")
+    for i in range(3):
+        s_i = st.copy(**{k: getattr(st, k).clone() for k in fields})
+        reqs = [Request(state=s_i, forcing=fo, geometry=ge, date=da, rid=i)]
+        server.run(reqs, realtime=False)
+        torch.cuda.synchronize()
+        assert len(server._members) == 1
+        del reqs, s_i
+        server.reset_stats()
+        gc.collect()
+        assert len(server._members) == 0, "a dropped state is still held by the server"
+
+
 def test_admission_respects_hbm_budget_and_overlaps():
     import torch
     from jcm_amd.serving import PhysicsServer
