@@ -12,7 +12,8 @@ land recipe of jcm_amd.synthetic (seed 1), inputs resident in HBM.
              u, v, ps and five parameters, cotangents N(0,1); two launches: the
              column kernel and the parameter-gradient sums).
   per_call   both calls made from Python one by one without graphs (the plain
-             per-call path: host overhead included).
+             per-call path: host overhead included), with the default
+             cooperative launch and with plain_launch=True.
   serving    a Poisson stream of requests of mixed resolution (T21, T31, T42,
              T85) and kind (forward, forward + VJP) through the continuous-
              batching loop (jcm_amd.serving) and through one call per request:
@@ -364,9 +365,17 @@ def main():
     # ---- plain per-call path (Python, no graphs)
     pc_f = per_call_ms(fwd, args.per_call_steps)
     pc_v = per_call_ms(vjp, args.per_call_steps)
+    # the same calls with plain instead of cooperative launches (plain_launch=True)
+    p_phys = SpeedyPhysics(device=dev, plain_launch=True)
+    p_vphys = SpeedyPhysics(parameters=vphys.parameters, device=dev, plain_launch=True)
+    pc_pf = per_call_ms(lambda: p_phys.compute_tendencies(*case), args.per_call_steps)
+    pc_pv = per_call_ms(lambda: p_vphys.compute_tendencies_vjp(*case, cot), args.per_call_steps)
     out["per_call"] = {"fwd_columns_per_s": ncol / (pc_f * 1e-3), "fwd_ms": pc_f,
                        "fwd_vjp_columns_per_s": ncol / (pc_v * 1e-3), "fwd_vjp_ms": pc_v,
-                       "speedup_graph_over_per_call": {"fwd": pc_f / ms_per_step, "fwd_vjp": pc_v / ms_vjp}}
+                       "speedup_graph_over_per_call": {"fwd": pc_f / ms_per_step, "fwd_vjp": pc_v / ms_vjp},
+                       "plain_launch": {"fwd_ms": pc_pf, "fwd_vjp_ms": pc_pv,
+                                        "fwd_columns_per_s": ncol / (pc_pf * 1e-3),
+                                        "fwd_vjp_columns_per_s": ncol / (pc_pv * 1e-3)}}
     out["large"] = large_leg(vphys, dev)
     if not args.no_serving:
         rates = [float(x) for x in args.serve_rates.split(",") if x]

@@ -54,7 +54,17 @@ enum {
  * unclamped q and dt = SpeedyDate.dt_seconds (verify_tendencies, :321-342). */
 #define SPEEDY_FLAG_VERIFY_STATE 0x2u
 #define SPEEDY_FLAG_VERIFY_TENDENCIES 0x4u
-#define SPEEDY_FLAGS_ALL 0x7u
+/* PLAIN_LAUNCH (no counterpart in the reference): an uncaptured call whose grid
+ * fits on the GPU at once makes a plain launch of the single-launch kernel
+ * instead of a cooperative one, as calls under graph capture always do.  The
+ * cooperative launch guarantees that all workgroups are resident together,
+ * which the kernel's grid-wide level-set exchange needs; it costs about 10 us
+ * per call on the GPU at T85.  With this flag, co-residency holds only when no
+ * other work shares the GPU.  The exchange's wait is bounded, so the kernel
+ * always ends, but if it gave up, speedy_workspace_status reports it and the
+ * call's results are invalid.  Ignored by speedy_fwd_batch. */
+#define SPEEDY_FLAG_PLAIN_LAUNCH 0x8u
+#define SPEEDY_FLAGS_ALL 0xFu
 
 /* term_mask bit i selects the tendency of term i (speedy_physics.py:67-80)
  * for the sum; every term always runs, so the data handed between terms are

@@ -1025,14 +1025,15 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
   hipError_t e = hipSuccess;
   if (fused) {
     const int slot = timer_begin(s, kind);
-    if (cs == hipStreamCaptureStatusNone) {
+    if (cs == hipStreamCaptureStatusNone && !(a.flags & SPEEDY_FLAG_PLAIN_LAUNCH)) {
       void* args[] = {const_cast<KArgs*>(&a)};
       e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&fwd_kernel<KX, DIAG, true, VJP, EXTRA>),
                                      dim3(a.nblk),
                                      dim3(kBlock), args, 0, s);
     } else {
-      // Graph capture: a plain launch of a grid that fits on the device; the
-      // level-set poll is bounded, so the kernel terminates in every case.
+      // Graph capture (or SPEEDY_FLAG_PLAIN_LAUNCH): a plain launch of a grid
+      // that fits on the device; the level-set poll is bounded, so the kernel
+      // terminates in every case.
       hipLaunchKernelGGL((fwd_kernel<KX, DIAG, true, VJP, EXTRA>), dim3(a.nblk), dim3(kBlock), 0, s, a);
       e = hipGetLastError();
     }

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 FLAG_LFLUXLAND = 0x1
 FLAG_VERIFY_STATE = 0x2
 FLAG_VERIFY_TENDENCIES = 0x4
+FLAG_PLAIN_LAUNCH = 0x8
 TERMS = ("set_physics_flags", "set_forcing", "spec_hum_to_rel_hum", "get_convection_tendencies",
          "get_large_scale_condensation_tendencies", "get_clouds", "get_shortwave_rad_fluxes",
          "get_downward_longwave_rad_fluxes", "get_surface_fluxes", "get_upward_longwave_rad_fluxes",

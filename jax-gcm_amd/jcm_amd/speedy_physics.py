@@ -144,9 +144,16 @@ class SpeedyPhysics(Physics):
     """SPEEDY physics (speedy_physics.py:34-133) fused into gfx950 kernels."""
 
     def __init__(self, parameters: Parameters | None = None, checkpoint_terms: bool = True,
-                 with_diagnostics: bool = False, device: str | torch.device | None = None):
+                 with_diagnostics: bool = False, device: str | torch.device | None = None,
+                 plain_launch: bool = False):
+        """plain_launch: uncaptured single-launch calls use a plain launch instead of a
+        cooperative one (about 10 us less per call at T85).  Only for a process that
+        has the GPU to itself: the grid-wide level-set exchange then relies on all
+        workgroups being resident together, and barrier_status() reports a call
+        whose exchange gave up (its results are invalid)."""
         self.parameters = parameters if parameters is not None else Parameters.default()
         self.with_diagnostics = with_diagnostics
+        self._launch_flags = _native.FLAG_PLAIN_LAUNCH if plain_launch else 0
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device()) \
             if torch.cuda.is_available() else None
         self._lib = _native.load()
@@ -325,7 +332,8 @@ class SpeedyPhysics(Physics):
             self.last_budgets = torch.empty((nmember, _native.NBUDGET), dtype=torch.float64, device=self.device)
             f.budget = self.last_budgets.data_ptr()
         ws = self.workspace(ix, il, nmember)
-        flags = (_native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0) | (VERIFY_FLAGS if verify else 0)
+        flags = ((_native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0) | (VERIFY_FLAGS if verify else 0)
+                 | self._launch_flags)
         rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(self._vertical(geometry)), ctypes.byref(self._params()),
                                   ctypes.byref(date.to_abi()), ix, il, nmember, flags, term_mask, level_set,
                                   ws.data_ptr(), ws.numel(), self._stream())
@@ -383,7 +391,7 @@ class SpeedyPhysics(Physics):
         self._require_gpu()
         keep: list = []
         f, adj, outputs, (ix, il, nmember), flags = self.prepare_vjp(state, forcing, geometry, date, cotangent, keep)
-        flags |= VERIFY_FLAGS if verify else 0
+        flags |= (VERIFY_FLAGS if verify else 0) | self._launch_flags
         ws = self.workspace(ix, il, nmember)
         rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(self._vertical(geometry)),
                                       ctypes.byref(self._params()), ctypes.byref(date.to_abi()), ix, il, nmember,

@@ -223,3 +223,31 @@ def test_vjp_with_fused_verify_steps(phys):
     _, g64 = H.run_host_vjp(case, params, cot, level_set=S, extra_flags=vf)
     _, g32 = H.run_host_vjp(case, params, cot, level_set=S, extra_flags=vf, dtype=np.float32)
     _check_against_host(g, prm, g64, "verify", g32)
+
+
+@pytest.mark.parametrize("nlon,nlat", [(256, 128), (64, 32)])
+def test_plain_launch_equals_cooperative(nlon, nlat):
+    """SPEEDY_FLAG_PLAIN_LAUNCH (SpeedyPhysics(plain_launch=True)): the single-launch
+    kernels started with a plain launch give the cooperative launch's tendencies,
+    state gradients and parameter gradients bit for bit, and the level-set exchange
+    did not give up."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    params = synthetic.draw_vjp_parameters(2)
+    coop = SpeedyPhysics(parameters=params)
+    plain = SpeedyPhysics(parameters=params, plain_launch=True)
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=29, land=True)
+    rng = np.random.default_rng(31)
+    cot = PhysicsTendency(*[torch.as_tensor(rng.standard_normal(st.temperature.shape).astype(np.float32),
+                                            device="cuda") for _ in range(4)])
+    outs = []
+    for phys in (coop, plain):
+        t, _ = phys.compute_tendencies(st, fo, ge, da)
+        tv, sg, pg = phys.compute_tendencies_vjp(st, fo, ge, da, cot)
+        torch.cuda.synchronize()
+        assert phys.barrier_status(nlon, nlat, 1) == 0, "level-set wait gave up"
+        outs.append(([_np(getattr(t, f)) for f in FIELDS] + [_np(getattr(tv, f)) for f in FIELDS]
+                     + [_np(getattr(sg, f)) for f in FIELDS + ("normalized_surface_pressure",)]
+                     + [_np(pg[k]) for k in sorted(pg)]))
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
