@@ -119,3 +119,24 @@ def test_climatology_rows_layout():
         assert tuple(rows.shape) == (9, 24) and rows.is_contiguous()
         for d in range(9):
             np.testing.assert_array_equal(rows[d].numpy(), arr[:, :, d].reshape(-1))
+
+
+def test_header_constants_match_host():
+    """Every numeric #define of include/speedy_abi.h that the Python host mirrors
+    (SPEEDY_X -> _native.X) has the same value; the flag mask is the OR of the flags."""
+    text = open(os.path.join(ROOT, "include", "speedy_abi.h")).read()
+    defs = {m.group(1): int(m.group(2), 0)
+            for m in re.finditer(r"^#define SPEEDY_(\w+) (0x[0-9A-Fa-f]+|\d+)u?\b", text, re.M)}
+    mirrored = {k: v for k, v in defs.items() if hasattr(_native, k)}
+    assert {"ABI_VERSION", "KX_MAX", "NBUDGET", "BATCH_MAX_SEGMENTS", "FLAG_LFLUXLAND", "FLAG_VERIFY_STATE",
+            "FLAG_VERIFY_TENDENCIES", "FLAG_PLAIN_LAUNCH"} <= set(mirrored)
+    for k, v in mirrored.items():
+        assert getattr(_native, k) == v, k
+    flags = [v for k, v in defs.items() if k.startswith("FLAG_")]
+    all_flags = 0
+    for v in flags:
+        assert v & (v - 1) == 0 and not (v & all_flags), "flags are distinct single bits"
+        all_flags |= v
+    assert defs["FLAGS_ALL"] == all_flags
+    nterms = len(set(re.findall(r"\bSPEEDY_TERM_(\w+)", text)))
+    assert len(_native.TERMS) == nterms and _native.TERMS_ALL == (1 << nterms) - 1
