@@ -260,10 +260,10 @@ def large_leg(phys, dev, reps=5):
     ms = sum(per_kind.values())
     achieved = FWD_BYTES_PER_COL(kx) * ncol / (per_kind["fwd"] * 1e-3) / 1e9
     out["fwd"] = {"columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
-                  "roofline": {"bound": "hbm", "kernel": "speedy::fwd_only_batch_kernel<8>", "achieved": achieved,
+                  "roofline": {"bound": "hbm", "kernel": "speedy::fwd_plain_batch_kernel<8>", "achieved": achieved,
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                               "traffic": profile_traffic("fwd_only_batch_kernel_T425"),
-                               "valu": profile_valu("speedy::fwd_only_batch_kernel<8>")}}
+                               "traffic": profile_traffic("fwd_plain_batch_kernel_T425"),
+                               "valu": profile_valu("speedy::fwd_plain_batch_kernel<8>")}}
     return out
 
 

@@ -319,6 +319,26 @@ def test_fwd_diagnostics_narrow_members():
             assert e32 <= 2e-5 or e64 <= max(2e-5, 2 * eref), f"member {m} {k}: {e32:.3e} (fp64 {e64:.3e})"
 
 
+@pytest.mark.parametrize("nlon,nlat,nmember", [(256, 128, 3), (16, 8, 5)])
+def test_fwd_batch_route_plain_kernel_equals_extra(nlon, nlat, nmember):
+    """One-segment batch forwards without the verify steps or budgets run
+    fwd_plain_batch_kernel (the forward without that optional code); with budgets
+    they run fwd_only_batch_kernel, which test_fwd_one_segment_batch_route holds to
+    the oracle.  The tendencies agree bit for bit."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=37, land=True, nmember=nmember, member_t_sigma=0.5)
+    q = np.array(st.specific_humidity)
+    q[1::2] *= np.float32(1.15)
+    case = (st.copy(specific_humidity=q), fo, ge, da)
+    t_plain, _ = phys.compute_tendencies(*case)
+    t_extra, _ = phys.compute_tendencies(*case, budgets=True)
+    torch.cuda.synchronize()
+    for f in FIELDS:
+        np.testing.assert_array_equal(_np(getattr(t_plain, f)), _np(getattr(t_extra, f)), err_msg=f)
+
+
 def test_fwd_repeated_calls_reset_level_set(phys):
     """The level set of one call does not leak into the next (self-resetting workspace)."""
     import torch
