@@ -330,21 +330,39 @@ __device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint
   const uint32_t nm = divq(c_end - 1u, a.dcol) - m_first + 1u;
   for (uint32_t i = threadIdx.x; i < nm; i += kBlock) s_or[i] = 0u;
   __syncthreads();
-  for (uint32_t gid = c_begin + threadIdx.x; gid < c_end; gid += kBlock) {
-    const uint32_t m = divq(gid, a.dcol);
-    const uint32_t c = gid - m * a.ncol;
-    ColumnIn<float, KX> col;
-    const size_t ncol = a.ncol;
-    const size_t base3 = (size_t)m * KX * ncol + c;
+  // kLsCols columns per thread per pass: all their loads issue before the first
+  // cloud top is computed (at two waves per SIMD one column's loads leave the
+  // memory latency exposed).  Columns past the chunk load column c_begin's
+  // words (in bounds) and are not ORed.
+  constexpr uint32_t kLsCols = 4;
+  const size_t ncol = a.ncol;
+  for (uint32_t g0 = c_begin + threadIdx.x; g0 < c_end; g0 += kLsCols * kBlock) {
+    ColumnIn<float, KX> col[kLsCols];
+    uint32_t mcol[kLsCols];
 #pragma unroll
-    for (int k = 1; k < KX; ++k) {
-      col.T[k] = a.f.t[base3 + k * ncol];
-      col.Q[k] = a.f.q[base3 + k * ncol];
-      col.PHI[k] = a.f.phi[base3 + k * ncol];
-      if (a.flags & SPEEDY_FLAG_VERIFY_STATE) col.Q[k] = verify_q(col.Q[k]);
+    for (uint32_t u = 0; u < kLsCols; ++u) {
+      const uint32_t gid = g0 + u * kBlock < c_end ? g0 + u * kBlock : c_begin;
+      const uint32_t m = divq(gid, a.dcol);
+      const uint32_t c = gid - m * a.ncol;
+      const size_t base3 = (size_t)m * KX * ncol + c;
+      mcol[u] = m;
+#pragma unroll
+      for (int k = 1; k < KX; ++k) {
+        col[u].T[k] = a.f.t[base3 + k * ncol];
+        col[u].Q[k] = a.f.q[base3 + k * ncol];
+        col[u].PHI[k] = a.f.phi[base3 + k * ncol];
+      }
+      col[u].ps = a.f.ps[(size_t)m * ncol + c];
     }
-    col.ps = a.f.ps[(size_t)m * ncol + c];
-    atomicOr(&s_or[m - m_first], clip_bit(cloud_top<float, KX>(col, a.p, a.v, a.u), KX));
+#pragma unroll
+    for (uint32_t u = 0; u < kLsCols; ++u) {
+      if (a.flags & SPEEDY_FLAG_VERIFY_STATE) {
+#pragma unroll
+        for (int k = 1; k < KX; ++k) col[u].Q[k] = verify_q(col[u].Q[k]);
+      }
+      if (g0 + u * kBlock < c_end)
+        atomicOr(&s_or[mcol[u] - m_first], clip_bit(cloud_top<float, KX>(col[u], a.p, a.v, a.u), KX));
+    }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[bid * a.jmax + i] = s_or[i];
