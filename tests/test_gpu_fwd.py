@@ -339,6 +339,37 @@ def test_fwd_batch_route_plain_kernel_equals_extra(nlon, nlat, nmember):
         np.testing.assert_array_equal(_np(getattr(t_plain, f)), _np(getattr(t_extra, f)), err_msg=f)
 
 
+@pytest.mark.parametrize("kx", [7, 16])
+def test_fwd_batch_route_other_level_counts(kx):
+    """Members narrower than a workgroup (16 x 8 columns, 5 members) at 7 and 16
+    levels through the batch kernels without the optional code
+    (fwd_plain_batch_kernel<7>, <16>): every member's level set and tendencies
+    against its oracle run, and bit for bit against the same call with budgets
+    (fwd_only_batch_kernel)."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    nlon, nlat, nmember = 16, 8, 5
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, kx, seed=41, land=True, nmember=nmember, member_t_sigma=0.5)
+    q = np.array(st.specific_humidity)
+    q[1::2] *= np.float32(1.15)
+    case = (st.copy(specific_humidity=q), fo, ge, da)
+    t_plain, _ = phys.compute_tendencies(*case)
+    torch.cuda.synchronize()
+    S = phys.level_sets(nlon, nlat, nmember)
+    t_extra, _ = phys.compute_tendencies(*case, budgets=True)
+    torch.cuda.synchronize()
+    for m in range(nmember):
+        ref, _, S_ref = H.run_oracle(*case, member=m)
+        ref64, _, _ = H.run_oracle(*case, member=m, dtype=np.float64)
+        assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
+        for f in FIELDS:
+            ok, e32, e64, eref = H.parity(_np(getattr(t_plain, f)[m]), getattr(ref, f), getattr(ref64, f))
+            assert ok, f"kx {kx} member {m} {f}: {e32:.3e} (fp64 {e64:.2e}, oracle {eref:.2e})"
+    for f in FIELDS:
+        np.testing.assert_array_equal(_np(getattr(t_plain, f)), _np(getattr(t_extra, f)), err_msg=f)
+
+
 def test_fwd_repeated_calls_reset_level_set(phys):
     """The level set of one call does not leak into the next (self-resetting workspace)."""
     import torch
