@@ -288,16 +288,17 @@ def test_fwd_diagnostics_batch_route_equals_fused(nlon, nlat):
     print(f"{nlon}x{nlat}: batch route == fused bit for bit: {identical}", flush=True)
 
 
-def test_fwd_diagnostics_narrow_members():
+@pytest.mark.parametrize("kx", [8, 7, 16])
+def test_fwd_diagnostics_narrow_members(kx):
     """Members narrower than a workgroup (16 x 8 columns, 5 members) with
-    diagnostics: level-set launch, then fwd_diag_batch_kernel.  Per member: level
-    set, tendencies and the PhysicsData subset against the member's oracle run
-    (diagnostic fields bound as in test_fwd_diagnostics)."""
+    diagnostics: level-set launch, then fwd_diag_batch_kernel, at 8, 7 and 16
+    levels.  Per member: level set, tendencies and the PhysicsData subset against
+    the member's oracle run (diagnostic fields bound as in test_fwd_diagnostics)."""
     import torch
     from jcm_amd.speedy_physics import SpeedyPhysics
     phys = SpeedyPhysics(with_diagnostics=True)
     nlon, nlat, nmember = 16, 8, 5
-    st, fo, ge, da = synthetic.make_case(nlon, nlat, 8, seed=23, land=True, nmember=nmember, member_t_sigma=0.5)
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, kx, seed=23, land=True, nmember=nmember, member_t_sigma=0.5)
     q = np.array(st.specific_humidity)
     q[1::2] *= np.float32(1.15)
     case = (st.copy(specific_humidity=q), fo, ge, da)
