@@ -251,3 +251,29 @@ def test_plain_launch_equals_cooperative(nlon, nlat):
                      + [_np(pg[k]) for k in sorted(pg)]))
     for a, b in zip(*outs):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("kx", [7, 16])
+def test_vjp_batch_route_other_level_counts(phys, kx):
+    """Members narrower than a workgroup (16 x 8 columns, 5 members) at 7 and 16
+    levels run the forward + VJP as a one-segment batch (fwd_batch_kernel<7>,
+    <16>): each member's level set, state gradients and parameter-gradient sums
+    against its own float64 adjoint (the bar of test_vjp_members_stacked)."""
+    nlon, nlat, nmember = 16, 8, 5
+    st, fo, ge, da = synthetic.make_case(nlon, nlat, kx, seed=43, land=True, nmember=nmember, member_t_sigma=1.0)
+    q = st.specific_humidity.copy()
+    q[1::2] *= np.float32(1.12)
+    st = st.copy(specific_humidity=q)
+    params = synthetic.draw_vjp_parameters(2)
+    cot = _cot((nmember, kx, nlon, nlat), 5)
+    _, g, prm = _gpu_vjp(phys, (st, fo, ge, da), params, cot)
+    S = phys.level_sets(nlon, nlat, nmember)
+    pick = lambda a, m: np.asarray(a)[m]
+    for m in range(nmember):
+        sm = st.copy(**{k: pick(getattr(st, k), m) for k in ("u_wind", "v_wind", "temperature", "specific_humidity",
+                                                             "geopotential", "normalized_surface_pressure")})
+        _, _, S_ref = H.run_oracle(sm, fo, ge, da, params=params)
+        assert S[m] == S_ref, (m, bin(S[m]), bin(S_ref))
+        _, g64 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m])
+        _, g32 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m], dtype=np.float32)
+        _check_against_host({k: v[m] for k, v in g.items()}, prm[m], g64, f"kx {kx} member {m}", g32)
