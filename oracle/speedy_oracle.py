@@ -1041,6 +1041,8 @@ PINNED_BY = {
     "get_surface_fluxes": "point values (surface_flux_test.py:67-106) and max/min/mean triples (:108-354)",
     "get_orog_land_sfc_drag": "surface_flux_test.py:355-364",
     "get_vertical_diffusion_tend": "column values (vertical_diffusion_test.py:25-50)",
-    "set_forcing": "closed form",
+    "set_forcing": "albedo limits, snow clamp and the float64 closed form of the CO2 trend (tests/test_oracle_pins.py)",
+    "verify_state": "physics_interface_test.py:46-64 properties, exact zeros, idempotence (tests/test_oracle_pins.py)",
+    "verify_tendencies": "element-by-element brute force and the non-negativity it guarantees (tests/test_oracle_pins.py)",
     "compute_tendencies": "composition of the pinned terms + conservation identities (column water budget)",
 }

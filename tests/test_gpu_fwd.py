@@ -130,7 +130,9 @@ def _data_fields(rdata):
         "shortwave_rad.fsol": rdata.shortwave_rad.fsol, "shortwave_rad.ozone": rdata.shortwave_rad.ozone,
         "shortwave_rad.ozupp": rdata.shortwave_rad.ozupp, "shortwave_rad.zenit": rdata.shortwave_rad.zenit,
         "shortwave_rad.stratz": rdata.shortwave_rad.stratz, "longwave_rad.ftop": rdata.longwave_rad.ftop,
-        "mod_radcon.albsfc": rdata.mod_radcon.albsfc, "surface_flux.rlds": rdata.surface_flux.rlds,
+        "mod_radcon.albsfc": rdata.mod_radcon.albsfc, "mod_radcon.alb_l": rdata.mod_radcon.alb_l,
+        "mod_radcon.alb_s": rdata.mod_radcon.alb_s, "mod_radcon.snowc": rdata.mod_radcon.snowc,
+        "surface_flux.rlds": rdata.surface_flux.rlds,
         "surface_flux.rlns": rdata.surface_flux.rlns, "surface_flux.tsfc": rdata.surface_flux.tsfc,
         "surface_flux.tskin": rdata.surface_flux.tskin, "surface_flux.t0": rdata.surface_flux.t0,
         "humidity.qsat": rdata.humidity.qsat, "humidity.rh": rdata.humidity.rh, "convection.se": rdata.convection.se,
@@ -166,6 +168,37 @@ def test_fwd_diagnostics(phys):
         worst = max(worst, min(e32, e64))
         assert e32 <= 2e-5 or e64 <= max(2e-5, 2 * eref), f"{k}: {e32:.3e} (fp64 {e64:.3e}, oracle {eref:.3e})"
     print(f"diagnostics: {len(checks)} fields, worst rel err {worst:.2e}", flush=True)
+
+
+def test_fwd_increase_co2_snow_and_sea_ice(phys):
+    """set_forcing (speedy/forcing.py:11-39) off its defaults: the CO2 trend on
+    (increase_co2, 80 years after co2_year_ref), snow cover above 1 in part of the
+    land (clamped to 1), sea ice between 0 and 1.  Tendencies meet the strict bar;
+    snowc, alb_l, alb_s, albsfc and the longwave transmissivities (tau2, band 1
+    carries ablco2) equal the float32 oracle's; the longwave tendencies move
+    against the default forcing, so the trend branch really ran."""
+    import torch
+    st, fo, ge, da = synthetic.make_case(96, 48, 8, seed=9, land=True)
+    rng = np.random.default_rng(11)
+    snow = np.asarray(fo.snowc_am, np.float32).copy()
+    snow[rng.random(snow.shape) < 0.3] = rng.uniform(1.0, 3.0)
+    sice = np.where(rng.random(snow.shape) < 0.4, rng.uniform(0.0, 1.0, snow.shape), 0.0).astype(np.float32)
+    case = (st, fo.copy(snowc_am=snow, sice_am=sice), ge, da.copy(model_year=2030, tyear=0.625))
+    params = Parameters.default().replace(forcing=dict(increase_co2=True))
+    tend, data, rdata = check_fwd(phys, case, params, label="co2 trend")
+    d = data.to_dict()
+    for k, ref in (("mod_radcon.snowc", rdata.mod_radcon.snowc), ("mod_radcon.alb_l", rdata.mod_radcon.alb_l),
+                   ("mod_radcon.alb_s", rdata.mod_radcon.alb_s), ("mod_radcon.albsfc", rdata.mod_radcon.albsfc)):
+        e = H.rel_inf(_np(d[k]), ref)
+        assert e <= 1e-6, f"{k}: {e:.3e}"
+    assert np.max(rdata.mod_radcon.snowc) == 1.0 and np.any(snow > 1.0)
+    for b in range(4):
+        e = H.rel_inf(_np(d[f"mod_radcon.tau2.{b}"]), rdata.mod_radcon.tau2[..., b])
+        assert e <= 1e-5, f"tau2.{b}: {e:.3e}"
+    phys.parameters = Parameters.default()
+    tend0, _ = phys.compute_tendencies(*case)
+    torch.cuda.synchronize()
+    assert H.rel_inf(_np(tend.temperature), _np(tend0.temperature)) > 1e-4
 
 
 def test_fwd_members_stacked(phys):

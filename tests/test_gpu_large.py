@@ -10,7 +10,10 @@ parameter-gradient reduction), one request on the continuous-batching server:
 Each case is served twice and the two results must be bit-identical.
 Checks, per member:
   * the level set equals the float32 oracle's over the whole grid (at most two
-    of the 64 members may differ, through a float32 tie in one column);
+    of the 64 members may differ, through a float32 tie in one column); every
+    member is checked, a member whose level set differs with the GPU's level
+    set injected into the oracle, and the tie columns behind the difference
+    are named from the GPU's own per-column cloud tops;
   * tendencies meet the forward bar of helpers.parity (float32 oracle on the
     whole grid, float64 oracle on a 4096-column subsample with the level set
     injected, helpers.subgrid);
@@ -125,11 +128,14 @@ def _check_gradients(label, g, prm, g64, g32):
         assert e <= max(GRAD_TOL, 3 * e32) or ref == 0.0 and prm[i] == 0.0, f"{label} {nm}: {e:.3e}"
 
 
-def _check_member(label, case, params, cot, tend, g, prm, S, member=None):
+def _check_member(label, case, params, cot, tend, g, prm, S, member=None, inject=False, directional=True):
+    """inject: the float32 oracle runs with the GPU's level set S (a member whose
+    free-running level set differs from the oracle's through a float32 tie, see
+    _name_tie_columns); otherwise the oracle's own level set must equal S."""
     kx, ix, il = case[2].nodal_shape
     pick = (lambda a: a[member]) if member is not None else (lambda a: a)
     # level set and tendencies against the float32 oracle on the whole grid
-    ref, _, S_ref = H.run_oracle(*case, params=params, member=member)
+    ref, _, S_ref = H.run_oracle(*case, params=params, member=member, level_set=S if inject else None)
     assert S == S_ref, f"{label}: level set {bin(S)} != {bin(S_ref)}"
     I, J = _subsample(ix, il, 4096, 7 if member is None else 7 + member)
     sub = H.subgrid(case, I, J, member)
@@ -148,6 +154,8 @@ def _check_member(label, case, params, cot, tend, g, prm, S, member=None):
     _, g64 = H.run_host_vjp(one, params, cm, level_set=S)
     _, g32 = H.run_host_vjp(one, params, cm, level_set=S, dtype=np.float32)
     _check_gradients(label, {k: pick(v) for k, v in g.items()}, pick(prm), g64, g32)
+    if not directional:
+        return
     # directional derivative on the subsample against the float64 oracle
     st, fo, ge, da = sub
     rng = np.random.default_rng(11)
@@ -243,11 +251,48 @@ def test_64_members_t31_16_levels():
     # every member's level set against the float32 oracle's; a member may differ
     # only through a float32 tie in one column's convection or condensation top
     # (the reference's float32 arithmetic and the kernel's round differently
-    # there; about one column in 3e5), at most two of 64
+    # there; about one column in 3e5), at most two of 64.  Such a member is
+    # compared with the GPU's level set injected into the oracle, and its tie
+    # columns are named.
     S_ref = [H.run_oracle(*case, params=params, member=m)[2] for m in range(64)]
     differ = [m for m in range(64) if S[m] != S_ref[m]]
     print(f"level sets differing from the float32 oracle: {[(m, bin(S[m]), bin(S_ref[m])) for m in differ]}",
           flush=True)
     assert len(differ) <= 2
-    for m in [m for m in (0, 1, 2, 3, 31, 62, 63) if m not in differ]:
-        _check_member(f"member {m}", case, params, cot, tend, g, prm, S[m], member=m)
+    for m in differ:
+        _name_tie_columns(case, params, m, S[m], S_ref[m])
+    for m in range(64):
+        _check_member(f"member {m}", case, params, cot, tend, g, prm, S[m], member=m, inject=m in differ,
+                      directional=m in (0, 1, 2, 3, 31, 62, 63) or m in differ)
+
+
+def _name_tie_columns(case, params, m, S_gpu, S_ref):
+    """The columns behind a level-set difference of member m: columns whose clipped
+    cloud top (shortwave_radiation.py:63-70) on the GPU (member m alone, with
+    diagnostics, which must reproduce the batch's level set) differs from the
+    float32 oracle's, at a level in S_gpu ^ S_ref.  The float64 oracle's and the
+    float32 host build's cloud tops at each named column show which side of the
+    tie each evaluation takes."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    kx, ix, il = case[2].nodal_shape
+    one = H.subgrid(case, np.arange(ix), np.arange(il), m)
+    phys = SpeedyPhysics(parameters=params, with_diagnostics=True)
+    _, data = phys.compute_tendencies(*one)
+    torch.cuda.synchronize()
+    assert phys.level_sets(ix, il, 1)[0] == S_gpu, f"member {m}: alone {bin(phys.level_sets(ix, il, 1)[0])}"
+    clip = lambda a: np.clip(np.asarray(a).astype(np.int64), 0, kx - 1)
+    c_gpu = clip(data.shortwave_rad.icltop.cpu().numpy())
+    ip_gpu = data.convection.iptop.cpu().numpy().astype(np.int64)
+    _, diag, S_host = H.run_host_column(one, params)
+    _, rdata, _ = H.run_oracle(*one, params=params)
+    _, rdata64, _ = H.run_oracle(*one, params=params, dtype=np.float64)
+    c_host, c_ref, c64 = clip(diag["icltop"]), clip(rdata.shortwave_rad.icltop), clip(rdata64.shortwave_rad.icltop)
+    bits = S_gpu ^ S_ref
+    cols = [(int(i), int(j)) for i, j in zip(*np.nonzero(c_gpu != c_ref))
+            if (bits >> int(c_gpu[i, j])) & 1 or (bits >> int(c_ref[i, j])) & 1]
+    for i, j in cols:
+        print(f"member {m}: tie column (lon {i}, lat {j}): cloud top GPU {c_gpu[i, j]}, float32 oracle "
+              f"{c_ref[i, j]}, float64 oracle {c64[i, j]}, float32 host build {c_host[i, j]}; top after "
+              f"condensation GPU {ip_gpu[i, j]}, float32 oracle {int(rdata.convection.iptop[i, j])}", flush=True)
+    assert 1 <= len(cols) <= 2, f"member {m}: {len(cols)} columns behind the level-set difference"

@@ -102,6 +102,30 @@ def test_vjp_matches_float64_adjoint(phys, name):
     assert phys.barrier_status(ix, il, 1) == 0
 
 
+def test_vjp_increase_co2(phys):
+    """The forward + VJP call with the CO2 trend on (speedy/forcing.py:31-33,
+    model_year 2030): gradients against the float64 adjoint, d/d co2_scale
+    included (its factor is the trended absorptivity)."""
+    name = "T21_land"
+    params = synthetic.draw_vjp_parameters(4).replace(forcing=dict(increase_co2=True, co2_scale=1.1))
+    st, fo, ge, da = make(name)
+    case = (st, fo, ge, da.copy(model_year=2030, tyear=0.625))
+    kx, ix, il = ge.nodal_shape
+    cot = _cot((kx, ix, il), 8)
+    tend, g, prm = _gpu_vjp(phys, case, params, cot)
+    S = phys.level_sets(ix, il, 1)[0]
+    _, _, S_ref = H.run_oracle(*case, params=params)
+    assert S == S_ref
+    _, g64 = H.run_host_vjp(case, params, cot, level_set=S)
+    _, g32 = H.run_host_vjp(case, params, cot, level_set=S, dtype=np.float32)
+    _check_against_host(g, prm, g64, "co2 trend", g32)
+    ref, _, _ = H.run_oracle(*case, params=params)
+    ref64, _, _ = H.run_oracle(*case, params=params, dtype=np.float64)
+    for f in FIELDS:
+        ok, e32, e64, eref = H.parity(_np(getattr(tend, f)), getattr(ref, f), getattr(ref64, f))
+        assert ok, f"co2 trend tendency {f}: {e32:.3e}"
+
+
 def test_vjp_tendencies_equal_forward_kernel(phys):
     """The forward part of the fused call is the forward kernel's code: same tendencies."""
     import torch

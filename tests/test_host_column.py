@@ -68,6 +68,29 @@ def test_host_column_matches_oracle(name):
         assert H.rel_inf(diag[k], ref) <= 1e-5, k
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_host_column_increase_co2(dtype):
+    """The CO2 trend of set_forcing (speedy/forcing.py:31-33) in the kernel's
+    uniforms: increase_co2 on, model_year 2030, co2_year_ref 1990.  float32 against
+    the float32 oracle (1e-5), float64 against the float64 oracle fed the same
+    float32 constants (1e-12); the result differs from the trend-free one."""
+    st, fo, ge, da = make("T21_land")
+    case = (st, fo, ge, da.copy(model_year=2030, tyear=0.625))
+    p = Parameters.default().replace(forcing=dict(increase_co2=True, co2_year_ref=1990))
+    outs, _, S = H.run_host_column(case, p, dtype=dtype)
+    if dtype == np.float32:
+        tend, _, S_ref = H.run_oracle(*case, params=p)
+        tol = H.REL_TOL
+    else:
+        tend, _, S_ref = H.run_oracle64_same_constants(case, p)
+        tol = F64_TOL
+    assert S == S_ref
+    for f, a in zip(FIELDS, outs):
+        assert H.rel_inf(a, getattr(tend, f)) <= tol, f
+    outs0, _, _ = H.run_host_column(case, Parameters.default(), dtype=dtype)
+    assert H.rel_inf(outs[2], outs0[2]) > 1e-4
+
+
 @pytest.mark.parametrize("term", [3, 4, 6, 8, 9, 10])
 def test_host_column_single_terms(term):
     """Each term's tendency alone (term_mask) against the oracle's.  A single term's
