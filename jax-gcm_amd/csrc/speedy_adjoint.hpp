@@ -47,6 +47,41 @@ struct ColumnAdj {
   R cloudc, clstr, qcloud, precnv, precls, rps;
 };
 
+// Materialize values at this point of the program: an empty asm that reads and
+// writes them.  Computations feeding them cannot be sunk past it, so a term's
+// (or a level's) reversal completes before the next one's recompute starts and
+// its intermediates die there.
+template <typename R>
+SPD_HD void settle(R& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
+#else
+  (void)x;
+#endif
+}
+
+// The fences are applied where the kernel then fits in registers (up to eight
+// levels: no scratch).  At sixteen levels the accumulators alone are ~100
+// values; forcing them into VGPRs at every fence makes the kernel spill around
+// the fences, and that build gave wrong forward tendencies in the batched
+// forward + VJP kernel on gfx950 (the host build of the same source is right).
+// Sixteen-level kernels therefore keep the unfenced form.
+template <int KX>
+constexpr bool kSettle = KX <= 8;
+
+template <typename R, int KX>
+SPD_HD void settle_adj(ColumnAdj<R, KX>& g) {
+  if constexpr (!kSettle<KX>) return;
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    settle(g.T[k]); settle(g.Q[k]); settle(g.qsat[k]); settle(g.rh[k]); settle(g.se[k]);
+  }
+  settle(g.ps); settle(g.u1); settle(g.v1);
+#pragma unroll
+  for (int i = 0; i < VJP_NPARAM; ++i) settle(g.prm[i]);
+  settle(g.cloudc); settle(g.clstr); settle(g.qcloud); settle(g.precnv); settle(g.precls); settle(g.rps);
+}
+
 template <typename R, int KX>
 SPD_HD void adj_zero(ColumnAdj<R, KX>& g) {
 #pragma unroll
@@ -113,13 +148,13 @@ SPD_HD R radset4_adj(R temp, R omeps, const R a_fb[4]) {
 // arrays stay live.  Gradients of the half-level temperatures go to a_Th.
 template <typename R, int KX>
 SPD_HD void lw_level_adj(int k, const R a_tau[4], R a_s0, R a_s1, const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a,
-                         const PhaseB<R, KX>& b, const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u,
+                         const R tau[4], const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u,
                          ColumnAdj<R, KX>& g, R a_Th[KX]) {
   R ab[4];
   lw_absorptivity<R, KX>(k, c, a, p, u, ab);
 #pragma unroll
   for (int bnd = 0; bnd < 4; ++bnd) {
-    const R ax = a_tau[bnd] * b.tauL[k][bnd];
+    const R ax = a_tau[bnd] * tau[bnd];
     g.ps += ax * (-(ab[bnd] * v.dhs[k]));
     const R a_ab = ax * (-(c.ps * v.dhs[k]));
     if (bnd == 0) {
@@ -171,16 +206,40 @@ SPD_HD void lw_level_adj(int k, const R a_tau[4], R a_s0, R a_s1, const ColumnIn
   }
 }
 
+// The column as level k's longwave recompute sees it: the values lw_level(k)
+// reads (T at k-1..k+1, q at k, ps) are passed through an empty asm that also
+// takes `dep`, the adjoint state entering level k, so the recompute cannot be
+// hoisted above the reversal of the previous level (which would keep every
+// level's recomputed values live at once).
+template <typename R, int KX>
+SPD_HD ColumnIn<R, KX> column_at_level(const ColumnIn<R, KX>& c, int k, R dep) {
+  ColumnIn<R, KX> cc = c;
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int j = k - 1; j <= k + 1; ++j)
+    if (j >= 0 && j < KX) asm volatile("" : "+v"(cc.T[j]) : "v"(dep));
+  asm volatile("" : "+v"(cc.Q[k]) : "v"(dep));
+  asm volatile("" : "+v"(cc.ps) : "v"(dep));
+#else
+  (void)k; (void)dep;
+#endif
+  return cc;
+}
+
 // ---------------------------------------------------------------- longwave up
-template <typename R, int KX, class FuAt>
+// Each level's transmissivities, emissions and band fractions are recomputed
+// here (lw_level) from the column, not taken from the forward.
+template <typename R, int KX, class Store>
 SPD_HD void lw_up_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b,
-                      const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, FuAt Fu,
+                      const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, const Store& Fu,
                       const R a_dfl[KX], R& a_rlus2, R a_flux_h[2], ColumnAdj<R, KX>& g, R a_Th[KX]) {
   const R ps = c.ps;
   const R stratc1 = u.eps1 * ps;
+  LwLevel<R> L0, L1;
+  lw_level<R, KX>(0, c, a, p, v, u, L0);
+  lw_level<R, KX>(1, c, a, p, v, u, L1);
   // corrections of the two top levels
   const R a_c1 = -a_dfl[0], a_c2 = -a_dfl[1];
-  const R a_strat1 = a_c1 * v.dhs[0] * b.st4a0[0] + a_c2 * v.dhs[1] * b.st4a0[1];
+  const R a_strat1 = a_c1 * v.dhs[0] * L0.st0 + a_c2 * v.dhs[1] * L1.st0;
   g.ps += a_c1 * c.stratz + a_strat1 * u.eps1;
   // top level: dfl[0] += (F0 - ft0) + (F1 - ft1), F = Fu[1]
   R aF[4];
@@ -188,40 +247,48 @@ SPD_HD void lw_up_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const Ph
     R afb[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
     R atau[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
     R as0 = a_c1 * v.dhs[0] * stratc1, as1 = R(0.0);
-    R fb[4];
-    radset4(c.T[0], u.omeps, fb);
 #pragma unroll
     for (int bnd = 0; bnd < 2; ++bnd) {
       const R a_ft = -a_dfl[0];
-      const R tau = b.tauL[0][bnd], em = R(1.0) - tau;
+      const R tau = L0.tau[bnd], em = R(1.0) - tau;
       aF[bnd] = a_dfl[0] + a_ft * tau;
-      atau[bnd] = a_ft * (Fu(1, bnd) - fb[bnd] * (b.st4a0[0] - R(2.0) * em * b.st4a1[0]));
-      afb[bnd] = a_ft * em * (b.st4a0[0] - em * b.st4a1[0]);
-      as0 += a_ft * em * fb[bnd];
-      as1 += -a_ft * em * em * fb[bnd];
+      atau[bnd] = a_ft * (Fu.get(4 + bnd) - L0.fb[bnd] * (L0.st0 - R(2.0) * em * L0.st1));
+      afb[bnd] = a_ft * em * (L0.st0 - em * L0.st1);
+      as0 += a_ft * em * L0.fb[bnd];
+      as1 += -a_ft * em * em * L0.fb[bnd];
     }
     aF[2] = R(0.0); aF[3] = R(0.0);
     g.T[0] += radset4_adj(c.T[0], u.omeps, afb);
-    lw_level_adj<R, KX>(0, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
+    lw_level_adj<R, KX>(0, atau, as0, as1, c, a, L0.tau, p, v, u, g, a_Th);
   }
   // levels 1 .. KX-1 (the forward ran from KX-1 down to 1)
 #pragma unroll
   for (int k = 1; k < KX; ++k) {
-    R fb[4], afb[4], atau[4];
+    LwLevel<R> Lk;
+    if (k == 1) Lk = L1;
+    else lw_level<R, KX>(k, column_at_level<R, KX>(c, k, aF[0]), a, p, v, u, Lk);
+    R afb[4], atau[4];
     R as0 = (k == 1) ? a_c2 * v.dhs[1] * stratc1 : R(0.0), as1 = R(0.0);
-    radset4(c.T[k], u.omeps, fb);
+    const auto Fk = Fu.after(aF[0]);   // Fu[k+1] read when level k is reversed
 #pragma unroll
     for (int bnd = 0; bnd < 4; ++bnd) {
-      const R tau = b.tauL[k][bnd], em = R(1.0) - tau;
+      const R tau = Lk.tau[bnd], em = R(1.0) - tau;
       const R a = aF[bnd] - a_dfl[k];            // dfl[k] += sum Fu[k+1] - sum Fu[k]
-      atau[bnd] = a * (Fu(k + 1, bnd) - fb[bnd] * (b.st4a0[k] - R(2.0) * em * b.st4a1[k]));
-      afb[bnd] = a * em * (b.st4a0[k] - em * b.st4a1[k]);
-      as0 += a * em * fb[bnd];
-      as1 += -a * em * em * fb[bnd];
+      atau[bnd] = a * (Fk.get((k + 1) * 4 + bnd) - Lk.fb[bnd] * (Lk.st0 - R(2.0) * em * Lk.st1));
+      afb[bnd] = a * em * (Lk.st0 - em * Lk.st1);
+      as0 += a * em * Lk.fb[bnd];
+      as1 += -a * em * em * Lk.fb[bnd];
       aF[bnd] = a_dfl[k] + a * tau;
     }
     g.T[k] += radset4_adj(c.T[k], u.omeps, afb);
-    lw_level_adj<R, KX>(k, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
+    lw_level_adj<R, KX>(k, atau, as0, as1, c, a, Lk.tau, p, v, u, g, a_Th);
+    settle_adj<R, KX>(g);
+    if constexpr (kSettle<KX>) {
+#pragma unroll
+      for (int j = 0; j < KX; ++j) settle(a_Th[j]);
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) settle(aF[bnd]);
+    }
   }
   // surface source: Fu[KX] = fbt * rlus2 + refsfc * flux_h; dfl[KX-1] += epslw * rlus2
   R fbt[4];
@@ -507,50 +574,59 @@ SPD_HD void shortwave_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, cons
 }
 
 // ---------------------------------------------------------------- longwave down
-template <typename R, int KX, class FdAt>
-SPD_HD void lw_down_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b,
-                        const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, FdAt Fd,
-                        const R a_dfl[KX], R a_rlds, ColumnAdj<R, KX>& g, R a_Th[KX]) {
-  // rlds = sum_b emisfc * Fd[KX-1][b] + corlw; dfl[KX-1] -= corlw; corlw = corlw_u * st4a0[KX-1]
-  const R a_corlw = (a_rlds - a_dfl[KX - 1]) * u.corlw;
+// As lw_up_adj, each level's longwave quantities are recomputed (lw_level).
+template <typename R, int KX, class Store>
+SPD_HD void lw_down_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
+                        const SpeedyVertical& v, const Uniforms<R>& u, const Store& Fd, const R a_dfl[KX], R a_rlds,
+                        ColumnAdj<R, KX>& g, R a_Th[KX]) {
   R aF[4];
 #pragma unroll
   for (int bnd = 0; bnd < 4; ++bnd) aF[bnd] = a_rlds * p.emisfc;
 #pragma unroll
   for (int k = KX - 1; k >= 1; --k) {
-    R fb[4], afb[4], atau[4];
-    R as0 = (k == KX - 1) ? a_corlw : R(0.0), as1 = R(0.0);
-    radset4(c.T[k], u.omeps, fb);
+    LwLevel<R> Lk;
+    lw_level<R, KX>(k, column_at_level<R, KX>(c, k, aF[0]), a, p, v, u, Lk);
+    R afb[4], atau[4];
+    // rlds = sum_b emisfc * Fd[KX-1][b] + corlw; dfl[KX-1] -= corlw; corlw = corlw_u * st4a0[KX-1]
+    R as0 = (k == KX - 1) ? (a_rlds - a_dfl[KX - 1]) * u.corlw : R(0.0), as1 = R(0.0);
+    const auto Fk = Fd.after(aF[0]);   // Fd[k-1] read when level k is reversed
 #pragma unroll
     for (int bnd = 0; bnd < 4; ++bnd) {
-      const R tau = b.tauL[k][bnd], em = R(1.0) - tau;
+      const R tau = Lk.tau[bnd], em = R(1.0) - tau;
       const R a = aF[bnd] - a_dfl[k];           // dfl[k] = sum Fd[k-1] - sum Fd[k]
-      atau[bnd] = a * (Fd(k - 1, bnd) - fb[bnd] * (b.st4a0[k] + R(2.0) * em * b.st4a1[k]));
-      afb[bnd] = a * em * (b.st4a0[k] + em * b.st4a1[k]);
-      as0 += a * em * fb[bnd];
-      as1 += a * em * em * fb[bnd];
+      atau[bnd] = a * (Fk.get((k - 1) * 4 + bnd) - Lk.fb[bnd] * (Lk.st0 + R(2.0) * em * Lk.st1));
+      afb[bnd] = a * em * (Lk.st0 + em * Lk.st1);
+      as0 += a * em * Lk.fb[bnd];
+      as1 += a * em * em * Lk.fb[bnd];
       aF[bnd] = a_dfl[k] + a * tau;
     }
     g.T[k] += radset4_adj(c.T[k], u.omeps, afb);
-    lw_level_adj<R, KX>(k, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
+    lw_level_adj<R, KX>(k, atau, as0, as1, c, a, Lk.tau, p, v, u, g, a_Th);
+    settle_adj<R, KX>(g);
+    if constexpr (kSettle<KX>) {
+#pragma unroll
+      for (int j = 0; j < KX; ++j) settle(a_Th[j]);
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) settle(aF[bnd]);
+    }
   }
   {
-    R fb[4];
-    radset4(c.T[0], u.omeps, fb);
+    LwLevel<R> L0;
+    lw_level<R, KX>(0, column_at_level<R, KX>(c, 0, aF[0]), a, p, v, u, L0);
     R afb[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
     R atau[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
     R as0 = R(0.0), as1 = R(0.0);
 #pragma unroll
     for (int bnd = 0; bnd < 2; ++bnd) {
-      const R tau = b.tauL[0][bnd], em = R(1.0) - tau;
+      const R tau = L0.tau[bnd], em = R(1.0) - tau;
       const R a = aF[bnd] - a_dfl[0];
-      atau[bnd] = -a * fb[bnd] * (b.st4a0[0] + R(2.0) * em * b.st4a1[0]);
-      afb[bnd] = a * em * (b.st4a0[0] + em * b.st4a1[0]);
-      as0 += a * em * fb[bnd];
-      as1 += a * em * em * fb[bnd];
+      atau[bnd] = -a * L0.fb[bnd] * (L0.st0 + R(2.0) * em * L0.st1);
+      afb[bnd] = a * em * (L0.st0 + em * L0.st1);
+      as0 += a * em * L0.fb[bnd];
+      as1 += a * em * em * L0.fb[bnd];
     }
     g.T[0] += radset4_adj(c.T[0], u.omeps, afb);
-    lw_level_adj<R, KX>(0, atau, as0, as1, c, a, b, p, v, u, g, a_Th);
+    lw_level_adj<R, KX>(0, atau, as0, as1, c, a, L0.tau, p, v, u, g, a_Th);
   }
 }
 
@@ -640,7 +716,7 @@ SPD_HD void clouds_adj(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const S
   g.Q[KX - 2] += g.qcloud;
   const R a_raw = a_cc * dmin_x(t.ccraw, R(1.0));
   const R mx = mmax(R(EPS), t.pr1);
-  const R a_pr1 = a_raw * p.wpcl * (R(0.5) / msqrt(mx)) * dmax_x(t.pr1, R(EPS));
+  const R a_pr1 = a_raw * p.wpcl * (R(0.5) / msqrt_1ulp(mx)) * dmax_x(t.pr1, R(EPS));
   const R a_sum = a_pr1 * dmin_x(R(86.4) * (a.precnv + a.precls), R(p.pmaxcl)) * R(86.4);
   g.precnv += a_sum;
   g.precls += a_sum;
@@ -895,19 +971,14 @@ struct ArrayStore {
   R v[N];
   SPD_HD void put(int i, R x) { v[i] = x; }
   SPD_HD R get(int i) const { return v[i]; }
+  SPD_HD const ArrayStore& after(R) const { return *this; }
 };
 
-// The whole column: given the forward's phase outputs (a: phase A, b: phase B
-// with the longwave-down divergence still in b.dfl and the hand-offs flux_h,
-// rlus2, tsfc) and the cotangents, accumulate the gradients into g.
-template <typename R, int KX, class Store>
-SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b, uint32_t S,
-                       const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, uint32_t flags,
-                       const ColumnCt<R, KX>& ct, ColumnAdj<R, KX>& g, Store& tape) {
-  adj_zero<R, KX>(g);
-  // The terms below recompute the forward from the column.  Hide the column's
-  // values from the optimizer so it recomputes instead of keeping every forward
-  // intermediate live across the adjoint (they are equal by construction).
+// A copy of the column whose values the optimizer cannot see through: a term
+// recomputed from it is not merged with the same computation elsewhere in the
+// kernel (which would keep that term's intermediates live in between).
+template <typename R, int KX>
+SPD_HD ColumnIn<R, KX> opaque_column(const ColumnIn<R, KX>& c_in) {
   ColumnIn<R, KX> c = c_in;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
@@ -918,6 +989,22 @@ SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, cons
   }
   asm volatile("" : "+v"(c.ps));
 #endif
+  return c;
+}
+
+// The whole column: given the forward's phase outputs (a: phase A; b: phase B's
+// scalars, the longwave-down divergence in b.dfl and the hand-offs flux_h, rlus2,
+// tsfc) and the cotangents, accumulate the gradients into g.  The longwave
+// transmissivities and emissions are recomputed where the longwave terms are
+// reversed, from their own opaque copies of the column, rather than read from b.
+template <typename R, int KX, class Store>
+SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, const PhaseB<R, KX>& b, uint32_t S,
+                       const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u, uint32_t flags,
+                       const ColumnCt<R, KX>& ct, ColumnAdj<R, KX>& g, Store& tape) {
+  adj_zero<R, KX>(g);
+  // The terms below recompute the forward from the column; none of the forward's
+  // intermediates stay live across the adjoint (they are equal by construction).
+  const ColumnIn<R, KX> c = opaque_column<R, KX>(c_in);
   const R rps = R(1.0) / c.ps;
   R a_dfl[KX], a_Th[KX];
 #pragma unroll
@@ -927,18 +1014,29 @@ SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, cons
   }
   R a_rlus2 = R(0.0), a_flux_h[2] = {R(0.0), R(0.0)}, a_rsds = R(0.0), a_rlds = R(0.0);
   {
-    PhaseB<R, KX> bb = b;
+    const ColumnIn<R, KX> cl = opaque_column<R, KX>(c);
+    PhaseB<R, KX> bb;
+    LwdTape<R, KX> et;
+    lw_transmissivity<R, KX>(cl, a, p, v, u, bb);
+    lw_emissions<R, KX>(cl, u, bb, et);
+#pragma unroll
+    for (int k = 0; k < KX; ++k) bb.dfl[k] = b.dfl[k];
+    bb.rlds = b.rlds;
     LwuTape<R, KX> lt;
     const R fh[4] = {b.flux_h[0], b.flux_h[1], R(0.0), R(0.0)};
-    lw_up<R, KX, false>(c, p, v, u, bb, b.rlus2, b.tsfc, fh, lt);
+    lw_up<R, KX, false>(cl, p, v, u, bb, b.rlus2, b.tsfc, fh, lt);
 #pragma unroll
     for (int k = 0; k < KX; ++k) g.rps += ct.dt[k] * lt.dfl[k] * v.grdscp[k];
 #pragma unroll
     for (int k = 1; k <= KX; ++k)
 #pragma unroll
       for (int bnd = 0; bnd < 4; ++bnd) tape.put(k * 4 + bnd, lt.Fu[k][bnd]);
-    lw_up_adj<R, KX>(c, a, b, p, v, u, [&](int k, int bnd) { return tape.get(k * 4 + bnd); }, a_dfl, a_rlus2,
-                     a_flux_h, g, a_Th);
+  }
+  {
+    const ColumnIn<R, KX> ca = opaque_column<R, KX>(c);
+    lw_up_adj<R, KX>(ca, a, b, p, v, u, tape, a_dfl, a_rlus2, a_flux_h, g, a_Th);
+    settle_adj<R, KX>(g);
+    if constexpr (kSettle<KX>) { settle(a_rlus2); settle(a_flux_h[0]); settle(a_flux_h[1]); }
   }
   {
     PhaseB<R, KX> bb = b;
@@ -946,6 +1044,8 @@ SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, cons
     R tsfc;
     surface<R, KX, false>(c, a, p, v, u, flags, bb, tsfc, st);
     surface_adj<R, KX>(c, a, b, p, v, u, st, ct, a_rlus2, a_rsds, a_rlds, g);
+    settle_adj<R, KX>(g);
+    if constexpr (kSettle<KX>) { settle(a_rsds); settle(a_rlds); }
   }
   {
     PhaseB<R, KX> bb = b;
@@ -953,27 +1053,36 @@ SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, cons
     R fh[4];
     shortwave<R, KX, false>(c, a, S, p, v, u, bb, fh, st);
     shortwave_adj<R, KX>(c, a, b, S, p, v, u, st, ct, a_flux_h, a_rsds, g);
+    settle_adj<R, KX>(g);
   }
   {
-    PhaseB<R, KX> bb = b;
+    const ColumnIn<R, KX> cl = opaque_column<R, KX>(c);
+    PhaseB<R, KX> bb;
     LwdTape<R, KX> lt;
-    lw_down<R, KX>(c, p, u, bb, lt);
+    lw_transmissivity<R, KX>(cl, a, p, v, u, bb);
+    lw_down<R, KX>(cl, p, u, bb, lt);
 #pragma unroll
     for (int k = 0; k < KX - 1; ++k)
 #pragma unroll
       for (int bnd = 0; bnd < 4; ++bnd) tape.put(k * 4 + bnd, lt.Fd[k][bnd]);
-    lw_down_adj<R, KX>(c, a, b, p, v, u, [&](int k, int bnd) { return tape.get(k * 4 + bnd); }, a_dfl, a_rlds, g,
-                       a_Th);
+  }
+  {
+    const ColumnIn<R, KX> ca = opaque_column<R, KX>(c);
+    lw_down_adj<R, KX>(ca, a, p, v, u, tape, a_dfl, a_rlds, g, a_Th);
+    settle_adj<R, KX>(g);
   }
   half_level_adj<R, KX>(u, a_Th, g);
   vdiff_adj<R, KX>(c, a, p, u, ct, g);
+  settle_adj<R, KX>(g);
   {
     PhaseA<R, KX> aa = a;
     CloudTape<R, KX> t;
     clouds<R, KX>(c, p, u, aa, t);
     clouds_adj<R, KX>(c, a, p, u, t, g);
+    settle_adj<R, KX>(g);
   }
   condensation_adj<R, KX>(c, a, u, ct, g);
+  settle_adj<R, KX>(g);
   {
     PhaseA<R, KX> aa = a;
     ConvTape<R, KX> t;

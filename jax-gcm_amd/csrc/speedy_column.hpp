@@ -44,6 +44,17 @@ SPD_HD float mcos(float x) { return cosf(x); }
 SPD_HD double mcos(double x) { return cos(x); }
 SPD_HD float masin(float x) { return asinf(x); }
 SPD_HD double masin(double x) { return asin(x); }
+// Square root where no discrete decision depends on the last bit: on the GPU the
+// hardware instruction (v_sqrt_f32, 1 ulp) instead of the correctly rounded
+// sequence; the same as msqrt on the host.
+SPD_HD float msqrt_1ulp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+SPD_HD double msqrt_1ulp(double x) { return sqrt(x); }
 
 // Derivatives of max(x, y) and min(x, y) with respect to x, with JAX's rule
 // for ties (lax.max/lax.min JVP: half to each operand when x == y).
@@ -109,7 +120,9 @@ SPD_HD void zonal_column(const ZonalUniforms<R>& z, R sia, R coa, R out[6]) {
   // clip bounds +-(1-1e-9) are exactly +-1 in float32 (:446)
   const R ch0 = mmin(mmax(-z.sol_tdecl * sia / coa, -R(1.0)), R(1.0));
   const R h0 = macos(ch0);
-  const R sh0 = msin(h0);
+  // sin(arccos(x)) = sqrt((1 - x)(1 + x)), exactly; (1 - x) and (1 + x) lose
+  // nothing near |x| = 1, and a square root is far cheaper than a sine
+  const R sh0 = msqrt_1ulp((R(1.0) - ch0) * (R(1.0) + ch0));
   const R topsr = z.sol_scale * (h0 * sia * z.sol_sdecl + sh0 * coa * z.sol_cdecl);
   const R flat2 = R(1.5) * sia * sia - R(0.5);
   const R ozone0 = R(0.4 * 0.020) * (R(1.0) + z.oz_coz1 * sia + R(1.8) * flat2);
@@ -119,7 +132,7 @@ SPD_HD void zonal_column(const ZonalUniforms<R>& z, R sia, R coa, R out[6]) {
   out[2] = topsr * ozone0 * zenit;
   out[3] = zenit;
   out[4] = mmax(R(6.0) - topsr, R(0.0));
-  out[5] = msqrt(coa);
+  out[5] = msqrt_1ulp(coa);
 }
 
 // verify_state (physics_interface.py:305-319): where(q < 0, 0, q).
@@ -508,7 +521,7 @@ SPD_HD void clouds(const ColumnIn<R, KX>& c, const SpeedyParams& p, const Unifor
   }
   const R pr1 = mmin(p.pmaxcl, R(86.4) * (a.precnv + a.precls));
   const R cl2 = mmin(R(1.0), cmax * rrcl);
-  const R ccraw = p.wpcl * msqrt(mmax(R(EPS), pr1)) + sq(cl2);
+  const R ccraw = p.wpcl * msqrt_1ulp(mmax(R(EPS), pr1)) + sq(cl2);
   R cloudc = mmin(R(1.0), ccraw);
   if (cloudc != cloudc) cloudc = R(1.0);
   a.cloudc = cloudc;
@@ -670,6 +683,42 @@ SPD_HD void lw_transmissivity(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, 
   }
 }
 
+// One level's longwave quantities, recomputed from the column alone: the band
+// transmissivities (shortwave_radiation.py:177-200), the emissions st4a[k][0:2]
+// (longwave_radiation.py:48-68) and the radset band fractions of T[k]
+// (longwave_radiation.py:216-225).  The same expressions as lw_transmissivity
+// and lw_down, so the values are equal to theirs; the adjoint recomputes them per
+// level instead of keeping the forward's [KX][4] arrays live.
+template <typename R>
+struct LwLevel {
+  R tau[4], st0, st1, fb[4];
+};
+
+template <typename R, int KX>
+SPD_HD void lw_level(int k, const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
+                     const SpeedyVertical& v, const Uniforms<R>& u, LwLevel<R>& L) {
+  R ab[4];
+  lw_absorptivity<R, KX>(k, c, a, p, u, ab);
+#pragma unroll
+  for (int bnd = 0; bnd < 4; ++bnd) L.tau[bnd] = exp_tr(-ab[bnd] * c.ps * v.dhs[k]);
+  auto Th = [&](int j) { return c.T[j] + u.wvi1[j] * (c.T[j + 1] - c.T[j]); };
+  if (k == 0) {
+    const R s0 = R(0.75) * c.T[0] + R(0.25) * Th(0);
+    L.st0 = R(SBC) * pow4(s0);
+    L.st1 = R(0.0);
+  } else if (k == 1) {
+    const R s1 = R(0.50) * c.T[1] + R(0.25) * (Th(0) + Th(1));
+    L.st0 = R(SBC) * pow4(s1);
+    L.st1 = R(0.0);
+  } else {
+    const R grad = (k < KX - 1) ? R(0.5) * mmax(Th(k) - Th(k - 1), R(0.0)) : mmax(c.T[KX - 1] - Th(KX - 2), R(0.0));
+    const R st3a = R(SBC) * pow3(c.T[k]);
+    L.st0 = st3a * c.T[k];
+    L.st1 = R(4.0) * st3a * grad;
+  }
+  radset4(c.T[k], u.omeps, L.fb);
+}
+
 template <typename R, int KX>
 struct LwdTape {
   R Th[KX], grad[KX], s0, s1;
@@ -677,9 +726,9 @@ struct LwdTape {
 };
 
 // longwave down (longwave_radiation.py:13-117)
+// Half-level temperatures and emissions st4a (longwave_radiation.py:48-68).
 template <typename R, int KX>
-SPD_HD void lw_down(const ColumnIn<R, KX>& c, const SpeedyParams& p, const Uniforms<R>& u, PhaseB<R, KX>& b,
-                    LwdTape<R, KX>& t) {
+SPD_HD void lw_emissions(const ColumnIn<R, KX>& c, const Uniforms<R>& u, PhaseB<R, KX>& b, LwdTape<R, KX>& t) {
   R* Th = t.Th;
 #pragma unroll
   for (int k = 0; k < KX - 1; ++k) Th[k] = c.T[k] + u.wvi1[k] * (c.T[k + 1] - c.T[k]);
@@ -698,6 +747,12 @@ SPD_HD void lw_down(const ColumnIn<R, KX>& c, const SpeedyParams& p, const Unifo
     b.st4a0[k] = st3a * c.T[k];
     b.st4a1[k] = R(4.0) * st3a * grad[k];
   }
+}
+
+template <typename R, int KX>
+SPD_HD void lw_down(const ColumnIn<R, KX>& c, const SpeedyParams& p, const Uniforms<R>& u, PhaseB<R, KX>& b,
+                    LwdTape<R, KX>& t) {
+  lw_emissions<R, KX>(c, u, b, t);
   R Fd[4];
   {
     R fb[4];
@@ -926,7 +981,7 @@ SPD_HD void surface(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const Spee
       t1_1 = Tb;
     }
     const R t0 = t1_1 + c.fmask * (t1_0 - t1_1);
-    const R wind = msqrt(u0 * u0 + v0 * v0 + R(s.vgust) * s.vgust);
+    const R wind = msqrt_1ulp(u0 * u0 + v0 * v0 + R(s.vgust) * s.vgust);
     den0 = (R(P0) * ps / (R(RGAS) * t0)) * wind;
     tskin = c.stl + s.ctday * c.sqcoa * b.rsds * (R(1.0) - b.alb_l) * ps;
     t.t1_0 = t1_0; t.t1_1 = t1_1; t.t0 = t0; t.wind = wind; t.den0 = den0; t.tskin0 = tskin;

@@ -372,8 +372,16 @@ __device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint
 // One column's slice of an LDS array laid out [slot][thread] (conflict-free).
 struct LdsStore {
   float* base;
-  __device__ __forceinline__ void put(int i, float x) { base[i * kBlock] = x; }
-  __device__ __forceinline__ float get(int i) const { return base[i * kBlock]; }
+  int off = 0;
+  __device__ __forceinline__ void put(int i, float x) { base[i * kBlock + off] = x; }
+  __device__ __forceinline__ float get(int i) const { return base[i * kBlock + off]; }
+  // The same slots, read only after `dep` is computed (the offset passes through
+  // an empty asm that takes dep), so the reads are not hoisted above it.
+  __device__ __forceinline__ LdsStore after(float dep) const {
+    LdsStore t = *this;
+    asm volatile("" : "+v"(t.off) : "v"(dep));
+    return t;
+  }
 };
 
 template <int KX>
@@ -1048,7 +1056,12 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
   }
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(s, &cs);
-  const bool fused = a.level_set < 0 && (int64_t)a.nblk <= cap && a.ncol >= (uint32_t)kBlock;
+  // Sixteen-level forward + VJP calls always take the batch route: at sixteen
+  // levels no one-thread-per-column kernel fits the register file, and the fused
+  // sixteen-level forward + VJP kernel (3.3 KB of scratch per lane) gave wrong
+  // gradients on gfx950 in a case whose host build is right; the batch kernel
+  // passes every sixteen-level case (cause of the fused kernel's error not found).
+  const bool fused = a.level_set < 0 && (int64_t)a.nblk <= cap && a.ncol >= (uint32_t)kBlock && !(VJP && KX == 16);
   const int kind = VJP ? 2 : 1;
   hipError_t e = hipSuccess;
   if (fused) {

@@ -192,9 +192,15 @@ def test_fwd_increase_co2_snow_and_sea_ice(phys):
         e = H.rel_inf(_np(d[k]), ref)
         assert e <= 1e-6, f"{k}: {e:.3e}"
     assert np.max(rdata.mod_radcon.snowc) == 1.0 and np.any(snow > 1.0)
+    # transmissivities: the diagnostics bar (band 0 carries the cloud cover, whose
+    # sqrt of the precipitation magnifies float32 cancellation in precnv)
+    r64 = check_fwd.rdata64.mod_radcon.tau2
     for b in range(4):
-        e = H.rel_inf(_np(d[f"mod_radcon.tau2.{b}"]), rdata.mod_radcon.tau2[..., b])
-        assert e <= 1e-5, f"tau2.{b}: {e:.3e}"
+        g = _np(d[f"mod_radcon.tau2.{b}"])
+        e32, e64 = H.rel_inf(g, rdata.mod_radcon.tau2[..., b]), H.rel_inf(g, r64[..., b])
+        eref = H.rel_inf(rdata.mod_radcon.tau2[..., b], r64[..., b])
+        print(f"co2 trend tau2.{b}: {e32:.2e} (fp64 {e64:.2e}, oracle {eref:.2e})", flush=True)
+        assert e32 <= 2e-5 or e64 <= max(2e-5, 2 * eref), f"tau2.{b}: {e32:.3e} (fp64 {e64:.3e}, oracle {eref:.3e})"
     phys.parameters = Parameters.default()
     tend0, _ = phys.compute_tendencies(*case)
     torch.cuda.synchronize()
