@@ -185,6 +185,22 @@ int speedy_fwd(const SpeedyFields *fields, const SpeedyVertical *vert, const Spe
                uint32_t term_mask, int32_t level_set, void *workspace, size_t workspace_bytes,
                void *stream);
 
+/* Allocation-free speedy_fwd.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same forward and
+ * the same results as speedy_fwd, with the workspace taken from a static device
+ * array of speedy_noalloc_max_bytes() (4 MiB) per GPU, held until the process
+ * ends; a grid whose speedy_workspace_bytes exceeds it is rejected with
+ * SPEEDY_ERR_CAPACITY.  The caller passes inputs and outputs as for speedy_fwd;
+ * no hipMalloc, hipFree or allocating call is made.  Every call on one GPU uses
+ * that GPU's array: calls that overlap on one GPU (two streams, two threads)
+ * race.  The call always runs as a one-segment batch (two to four launches),
+ * which reads nothing an earlier call left in the array. */
+size_t speedy_noalloc_max_bytes(void);
+int speedy_fwd_noalloc(const SpeedyFields *fields, const SpeedyVertical *vert, const SpeedyParams *params,
+                       const SpeedyDate *date, int32_t nlon, int32_t nlat, int32_t nmember, uint32_t flags,
+                       uint32_t term_mask, int32_t level_set, void *stream);
+
 /* Fused forward + VJP: the forward tendencies (as speedy_fwd with every term)
  * and the cotangent-weighted gradients w.r.t. T, q, u, v, ps and five
  * parameters.  Replaces jax.vjp of compute_tendencies (speedy_physics.py:88-127)

@@ -40,6 +40,9 @@
 namespace speedy {
 
 constexpr int kBlock = 256;
+// Internal flag (above the public SPEEDY_FLAG_* bits): take the one-segment batch
+// route even where the fused launch could run (speedy_fwd_noalloc).
+constexpr uint32_t kFlagBatchRoute = 1u << 31;
 constexpr uint32_t kSpinLimit = 4u << 20;   // barrier wait bound (~seconds): a wave always exits
 
 struct Divisor {          // n / d for n < 2^32 by multiply-high + one correction step
@@ -1061,7 +1064,8 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
   // sixteen-level forward + VJP kernel (3.3 KB of scratch per lane) gave wrong
   // gradients on gfx950 in a case whose host build is right; the batch kernel
   // passes every sixteen-level case (cause of the fused kernel's error not found).
-  const bool fused = a.level_set < 0 && (int64_t)a.nblk <= cap && a.ncol >= (uint32_t)kBlock && !(VJP && KX == 16);
+  const bool fused = a.level_set < 0 && (int64_t)a.nblk <= cap && a.ncol >= (uint32_t)kBlock && !(VJP && KX == 16) &&
+                     !(a.flags & kFlagBatchRoute);
   const int kind = VJP ? 2 : 1;
   hipError_t e = hipSuccess;
   if (fused) {
@@ -1454,3 +1458,63 @@ int speedy_timer_read(float* ms_per_kind, int32_t* launches_per_kind, int32_t nk
 const char* speedy_last_error(void) { return g_last_error.c_str(); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free forward
+// speedy_fwd_noalloc: speedy_fwd with its workspace in this file-scope device
+// array instead of a caller's buffer, for callers that cannot allocate.  Each GPU
+// holds its own copy for the life of the process; its device address is looked
+// up once per device (hipGetSymbolAddress) and only that address is passed to
+// kernels.  Calls take the one-segment batch route, which reads no word of the
+// workspace that the same call has not written first (descriptor, level-set
+// words, partials), so the contents left by an earlier call, of any grid, are
+// never read and nothing needs clearing.  Calls that overlap on one GPU share
+// the array and race.
+namespace speedy {
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+}  // namespace speedy
+
+extern "C" size_t speedy_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int speedy_fwd_noalloc(const SpeedyFields* fields, const SpeedyVertical* vert, const SpeedyParams* params,
+                                  const SpeedyDate* date, int32_t nlon, int32_t nlat, int32_t nmember, uint32_t flags,
+                                  uint32_t term_mask, int32_t level_set, void* stream) {
+  const size_t need = speedy_workspace_bytes(nlon, nlat, nmember);
+  if (need == 0) return fail(SPEEDY_ERR_ARG, "speedy_fwd_noalloc: bad grid");
+  if (need > kNoAllocScratchBytes)
+    return fail(SPEEDY_ERR_CAPACITY, "speedy_fwd_noalloc: the call needs " + std::to_string(need) +
+                                         " bytes of workspace, above the " + std::to_string(kNoAllocScratchBytes) +
+                                         "-byte static scratch; use speedy_fwd with a caller workspace");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_noalloc: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(SPEEDY_ERR_CAPACITY, "speedy_fwd_noalloc: device index");
+  void* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_noalloc: scratch address: ") + hipGetErrorString(e));
+      g_noalloc_ptr[dev] = p;
+    }
+    ws = g_noalloc_ptr[dev];
+  }
+  KArgs a;
+  if (int rc = prepare(fields, vert, params, date, nlon, nlat, nmember, flags, term_mask, level_set, ws,
+                       kNoAllocScratchBytes, a))
+    return rc;
+  a.flags |= kFlagBatchRoute;
+  g_last_error.clear();
+  hipStream_t s = (hipStream_t)stream;
+  switch (vert->kx) {
+    case 7: return launch_fwd_kx<7>(a, s);
+    case 8: return launch_fwd_kx<8>(a, s);
+    default: return launch_fwd_kx<16>(a, s);
+  }
+}

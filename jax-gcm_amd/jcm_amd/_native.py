@@ -115,6 +115,13 @@ def load(path: str | None = None):
                                ctypes.POINTER(SpeedyParams), ctypes.POINTER(SpeedyDate), ctypes.c_int32,
                                ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
                                CFP, ctypes.c_size_t, CFP]
+    lib.speedy_noalloc_max_bytes.restype = ctypes.c_size_t
+    lib.speedy_noalloc_max_bytes.argtypes = []
+    lib.speedy_fwd_noalloc.restype = ctypes.c_int
+    lib.speedy_fwd_noalloc.argtypes = [ctypes.POINTER(SpeedyFields), ctypes.POINTER(SpeedyVertical),
+                                       ctypes.POINTER(SpeedyParams), ctypes.POINTER(SpeedyDate), ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_int32, CFP]
     lib.speedy_fwd_vjp.restype = ctypes.c_int
     lib.speedy_fwd_vjp.argtypes = [ctypes.POINTER(SpeedyFields), ctypes.POINTER(SpeedyAdjoint),
                                    ctypes.POINTER(SpeedyVertical), ctypes.POINTER(SpeedyParams),

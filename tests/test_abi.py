@@ -140,3 +140,23 @@ def test_header_constants_match_host():
     assert defs["FLAGS_ALL"] == all_flags
     nterms = len(set(re.findall(r"\bSPEEDY_TERM_(\w+)", text)))
     assert len(_native.TERMS) == nterms and _native.TERMS_ALL == (1 << nterms) - 1
+
+
+def test_noalloc_rejects_grids_above_its_scratch():
+    """speedy_fwd_noalloc checks the grid's workspace against its static scratch
+    before any GPU work: a grid that needs more is rejected with
+    SPEEDY_ERR_CAPACITY and a message naming the limit."""
+    lib = _native.load()
+    cap = lib.speedy_noalloc_max_bytes()
+    assert cap == 4 << 20
+    assert lib.speedy_workspace_bytes(1280, 640, 1) <= cap          # T425, one member: fits
+    assert lib.speedy_workspace_bytes(1280, 640, 16) > cap          # sixteen T425 members: does not
+    f = _native.SpeedyFields()
+    geo = Geometry.from_latitudes(__import__("numpy").linspace(-1.4, 1.4, 640), 1280, 8)
+    rc = lib.speedy_fwd_noalloc(ctypes.byref(f), ctypes.byref(geo.to_abi()), ctypes.byref(Parameters().to_abi()),
+                                ctypes.byref(_native.SpeedyDate()), 1280, 640, 16, 0, _native.TERMS_ALL, -1, None)
+    assert rc == 5, rc
+    assert b"static scratch" in lib.speedy_last_error()
+    rc = lib.speedy_fwd_noalloc(ctypes.byref(f), ctypes.byref(geo.to_abi()), ctypes.byref(Parameters().to_abi()),
+                                ctypes.byref(_native.SpeedyDate()), 0, 640, 1, 0, _native.TERMS_ALL, -1, None)
+    assert rc == 1, rc
