@@ -7,7 +7,9 @@ land recipe of jcm_amd.synthetic (seed 1), inputs resident in HBM.
              eleven SPEEDY terms, one fused kernel launch).  Steps are replayed
              from HIP graphs of --graph-calls calls so that host launch overhead
              does not hide device time; K steps are bracketed by
-             torch.cuda.synchronize() on both sides.
+             torch.cuda.synchronize() on both sides, and HIP events on the
+             replay stream time them (untimed replays queued just before keep
+             the GPU's clock ramp after setup out of short measurements).
   fwd_vjp    the same for compute_tendencies_vjp (fused forward + VJP w.r.t. T, q,
              u, v, ps and five parameters, cotangents N(0,1); two launches: the
              column kernel and the parameter-gradient sums).
@@ -21,9 +23,12 @@ land recipe of jcm_amd.synthetic (seed 1), inputs resident in HBM.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the fused forward
 kernel, one launch per step): its average duration is the graph-replayed step
-time measured with HIP events on the replay stream; `cpu_baseline` times the
-float32 NumPy oracle (the CPU restatement of the reference) on the same
-workload on one host core.
+time measured with HIP events on the replay stream; `achieved` counts the path's
+algorithmic bytes (328 B per column at 8 levels: u, v, T, q, phi at every
+level), `used_bytes` what the kernel needs (272 B: u and v at the lowest level
+only).  `cpu_baseline` times the kernel's own column routine built for the host
+with OpenMP on the host's cores (forward and forward + VJP, T85 and T31), and
+the float32 NumPy oracle on one core.
 """
 from __future__ import annotations
 
@@ -41,16 +46,30 @@ HBM_PEAK_GBS = 8000.0
 
 
 def FWD_BYTES_PER_COL(kx):
-    """Forward: reads T, q, phi at all levels, u and v at the lowest level, ps,
-    6 forcing and 3 geometry fields (3*kx + 12 floats); writes the 4 tendency
-    fields (4*kx floats)."""
-    return (3 * kx + 12 + 4 * kx) * 4
+    """Algorithmic bytes of the forward per column, as the path's contract counts
+    them: reads u, v, T, q, phi at all levels and ps (5*kx + 1 floats), 6 forcing
+    and 3 geometry fields; writes the 4 tendency fields (4*kx): (9*kx + 10) * 4,
+    328 B at 8 levels."""
+    return (9 * kx + 10) * 4
 
 
 def VJP_BYTES_PER_COL(kx):
-    """Forward + VJP: the forward's reads plus the cotangents that matter (dT, dq
-    at all levels, du, dv at the lowest: 2*kx + 2); writes the 4 tendency fields
-    and the 4 state gradients (8*kx) and dL/dps (1)."""
+    """Algorithmic bytes of the forward + VJP per column: the forward's reads plus
+    the 4 cotangent fields (4*kx); writes the 4 tendency fields, the 4 state
+    gradients (8*kx) and dL/dps (1): (17*kx + 11) * 4, 588 B at 8 levels."""
+    return (17 * kx + 11) * 4
+
+
+def FWD_USED_BYTES_PER_COL(kx):
+    """Bytes the forward kernel actually needs: u and v are read at the lowest
+    level only (only the surface stress uses them): (3*kx + 12 + 4*kx) * 4,
+    272 B at 8 levels."""
+    return (3 * kx + 12 + 4 * kx) * 4
+
+
+def VJP_USED_BYTES_PER_COL(kx):
+    """As FWD_USED_BYTES_PER_COL for the forward + VJP (cotangents du, dv at the
+    lowest level only): ((3*kx + 12) + (2*kx + 2) + (8*kx + 1)) * 4, 476 B."""
     return ((3 * kx + 12) + (2 * kx + 2) + (8 * kx + 1)) * 4
 
 
@@ -85,21 +104,64 @@ def to_device(case, dev):
     return st, fo, geometry, date
 
 
-def cpu_baseline(case, seconds):
-    """Float32 NumPy oracle on the same T85 workload, one host core."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import helpers
-    ncol = case[2].ncol
+def _timed(fn, seconds, max_n=100000):
     t0 = time.perf_counter()
     n = 0
     while True:
-        helpers.run_oracle(*case)
+        fn()
         n += 1
         el = time.perf_counter() - t0
-        if el >= seconds or n >= 1000:
-            break
-    return {"value": ncol * n / el, "unit": "columns/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full T85x8 forward calls ({n * ncol} columns) of the float32 NumPy oracle, {el:.1f} s"}
+        if el >= seconds or n >= max_n:
+            return n, el
+
+
+def _host_cpus():
+    """(threads the OpenMP leg uses, CPUs the host shows, CPU model)."""
+    ncpu = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", ncpu))
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, ncpu, model
+
+
+def cpu_baseline(case_t85, case_t31, seconds):
+    """CPU restatements of the path, timed on this host: the kernel's own column
+    routine compiled for the host with OpenMP over the columns (all threads the
+    host allows, OMP_NUM_THREADS) -- forward and forward + VJP at T85 and T31 --
+    and the float32 NumPy oracle on one core."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers
+    from jcm_amd import synthetic
+    threads, ncpu, model = _host_cpus()
+    omp = ctypes.CDLL(os.path.join(ROOT, "tests", "host_column", "libcolumn_host_omp.so"))
+    helpers.build_host_column = lambda: omp      # the OpenMP build of the same harness
+    legs = {}
+    params = synthetic.draw_vjp_parameters(2)
+    for name, case in (("T85", case_t85), ("T31", case_t31)):
+        kx, ix, il = case[2].nodal_shape
+        ncol = ix * il
+        rng = np.random.default_rng(2)
+        cot = [rng.standard_normal((kx, ix, il)).astype(np.float32) for _ in range(4)]
+        n, el = _timed(lambda: helpers.run_host_column(case, params), seconds / 4)
+        legs[f"fwd_{name}"] = ncol * n / el
+        n, el = _timed(lambda: helpers.run_host_vjp(case, params, cot, dtype=np.float32), seconds / 4)
+        legs[f"fwd_vjp_{name}"] = ncol * n / el
+    n, el = _timed(lambda: helpers.run_oracle(*case_t85), seconds / 2, max_n=1000)
+    numpy = case_t85[2].ncol * n / el
+    return {"value": legs["fwd_T85"], "unit": "columns/s", "cores": threads, "kind": "port",
+            "sample": f"the kernel's column routine built for the host (g++ -O2 -fopenmp, float32), T85x8 forward "
+                      f"calls for {seconds / 4:.1f} s on {threads} OpenMP threads of a {ncpu}-CPU host ({model})",
+            "legs_columns_per_s": legs, "host_cpus": ncpu, "cpu_model": model,
+            "numpy_oracle": {"value": numpy, "unit": "columns/s", "cores": 1,
+                             "sample": f"{n} T85x8 forward calls of the float32 NumPy oracle, {el:.1f} s"}}
 
 
 class GraphStep:
@@ -121,12 +183,22 @@ class GraphStep:
         self.graph.replay()
         torch.cuda.synchronize()
 
-    def time(self, reps):
-        """ms per call over reps replays, HIP events on the replaying stream."""
+    def time(self, reps, warm_ms=20.0):
+        """ms per call over reps replays, HIP events on the replaying stream.  Untimed
+        replays (about warm_ms of device work) are queued right ahead of the first
+        event: the GPU's clocks are up and the stream is busy when the timed replays
+        start, so a short measurement (one replay of a 20-call graph) sees the same
+        per-call time as a long one instead of the clock ramp after setup."""
         import torch
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        self.graph.replay()
+        torch.cuda.synchronize()
+        one = max(time.perf_counter() - w0, 1e-6)
+        for _ in range(max(1, int(warm_ms * 1e-3 / one))):
+            self.graph.replay()
         t0.record()
         for _ in range(reps):
             self.graph.replay()
@@ -243,10 +315,12 @@ def large_leg(phys, dev, reps=5):
     ms = sum(per_kind.values())
     kx = 8
     achieved = VJP_BYTES_PER_COL(kx) * ncol / (per_kind["fwd_vjp"] * 1e-3) / 1e9
+    used = VJP_USED_BYTES_PER_COL(kx) * ncol / (per_kind["fwd_vjp"] * 1e-3) / 1e9
     out = {"workload": "T425 1280x640 = 819200 columns x 8 levels, forward + VJP, batched entry (one request)",
            "columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
            "roofline": {"bound": "hbm", "kernel": "speedy::fwd_batch_kernel<8>", "achieved": achieved,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "used_bytes_frac": used / HBM_PEAK_GBS,
                         "traffic": profile_traffic("fwd_batch_kernel_T425"),
                         "valu": profile_valu("speedy::fwd_batch_kernel<8>")}}
     # the forward alone (forward-only batch kernel)
@@ -259,9 +333,11 @@ def large_leg(phys, dev, reps=5):
     per_kind = {k: v[0] / reps for k, v in tm.items() if v[1]}
     ms = sum(per_kind.values())
     achieved = FWD_BYTES_PER_COL(kx) * ncol / (per_kind["fwd"] * 1e-3) / 1e9
+    used = FWD_USED_BYTES_PER_COL(kx) * ncol / (per_kind["fwd"] * 1e-3) / 1e9
     out["fwd"] = {"columns_per_s": ncol / (ms * 1e-3), "ms_per_call": ms, "kernel_ms": per_kind,
                   "roofline": {"bound": "hbm", "kernel": "speedy::fwd_plain_batch_kernel<8>", "achieved": achieved,
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                               "used_bytes_frac": used / HBM_PEAK_GBS,
                                "traffic": profile_traffic("fwd_plain_batch_kernel_T425"),
                                "valu": profile_valu("speedy::fwd_plain_batch_kernel<8>")}}
     return out
@@ -336,10 +412,22 @@ def main():
     lib.speedy_timer_disable()
     launches = {k: v[1] for k, v in tm.items() if v[1]}
 
+    # per-kernel device time of the forward + VJP call (HIP events around each of
+    # its two kernels, plain calls): the VJP kernel without the parameter sums
+    lib.speedy_timer_enable(64)
+    for _ in range(20):
+        vjp()
+    torch.cuda.synchronize()
+    tm_v = _native.timer_read()
+    lib.speedy_timer_disable()
+    vjp_kernel_ms = {k: v[0] / v[1] for k, v in tm_v.items() if v[1]}
+
     bytes_fwd = FWD_BYTES_PER_COL(kx) * ncol
     achieved = bytes_fwd / (ms_per_step * 1e-3) / 1e9
+    used_fwd = FWD_USED_BYTES_PER_COL(kx) * ncol / (ms_per_step * 1e-3) / 1e9
     bytes_vjp = VJP_BYTES_PER_COL(kx) * ncol
     achieved_vjp = bytes_vjp / (ms_vjp * 1e-3) / 1e9
+    used_vjp = VJP_USED_BYTES_PER_COL(kx) * ncol / (ms_vjp * 1e-3) / 1e9
     traffic = profile_traffic("fwd_kernel_T85")
 
     out = {
@@ -352,11 +440,17 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "speedy::fwd_kernel<8, false, true, false, false>", "kernel_ms": ms_per_step,
                      "valu": profile_valu("speedy::fwd_kernel<8, false, true, false, false>"),
-                     "algorithmic_bytes_per_launch": bytes_fwd},
+                     "algorithmic_bytes_per_launch": bytes_fwd,
+                     "bytes_per_column": FWD_BYTES_PER_COL(kx),
+                     "used_bytes": {"bytes_per_column": FWD_USED_BYTES_PER_COL(kx), "achieved": used_fwd,
+                                    "frac": used_fwd / HBM_PEAK_GBS}},
         "fwd_vjp": {"value": ncol / (ms_vjp * 1e-3), "unit": "columns/s", "ms_per_step": ms_vjp,
-                    "launches_per_step": 2,
+                    "launches_per_step": 2, "kernel_ms": vjp_kernel_ms,
                     "roofline": {"bound": "hbm", "achieved": achieved_vjp, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": achieved_vjp / HBM_PEAK_GBS, "algorithmic_bytes_per_step": bytes_vjp,
+                                 "bytes_per_column": VJP_BYTES_PER_COL(kx),
+                                 "used_bytes": {"bytes_per_column": VJP_USED_BYTES_PER_COL(kx), "achieved": used_vjp,
+                                                "frac": used_vjp / HBM_PEAK_GBS},
                                  "traffic": profile_traffic("fwd_vjp_kernel_T85"),
                                  "valu": profile_valu("speedy::fwd_kernel<8, false, true, true, true>"),
                                  "kernel": "speedy::fwd_kernel<8, false, true, true, true> + param_reduce_kernel"}},
@@ -381,7 +475,8 @@ def main():
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         out["serving"] = serving_runs(vphys, dev, args.serve_requests, rates)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case_np, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(case_np, synthetic.make_case(96, 48, 8, seed=0, land=False),
+                                           args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -164,7 +164,10 @@ size_t speedy_workspace_bytes(int32_t nlon, int32_t nlat, int32_t nmember);
 int speedy_workspace_init(void *workspace, size_t workspace_bytes, void *stream);
 
 /* Status of a workspace: bit 0 is set if a call's grid barrier gave up
- * waiting (that call's results are invalid).  Reading clears it. */
+ * waiting (that call's results are invalid).  Reading a set bit clears it and
+ * re-zeroes the whole workspace (a fill kernel on `stream`): a failed wait
+ * leaves late blocks' level-set words and the generation word out of step, and
+ * the next call then starts from a clean workspace. */
 int speedy_workspace_status(void *workspace, size_t workspace_bytes, uint32_t *status, void *stream);
 
 /* Rows of the diagnostic output for kx levels, and their names as a

@@ -262,6 +262,8 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
 
 __device__ __forceinline__ uint32_t gen_tag(uint32_t g) { return (g & 0x7fffu) | 0x8000u; }
 
+
+
 __device__ __forceinline__ uint32_t clip_bit(int icl, int kx) {
   return 1u << (icl < 0 ? 0 : (icl > kx - 1 ? kx - 1 : icl));
 }
@@ -679,10 +681,6 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   }
 }
 
-template <int KX, bool DIAG, bool FUSED, bool VJP, bool EXTRA>
-__global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
-  fwd_body<KX, DIAG, FUSED, VJP, EXTRA>(a, blockIdx.x);
-}
 
 // Per-member parameter gradients from the partials, one block per member, in a
 // fixed order (deterministic).
@@ -779,6 +777,11 @@ __device__ __forceinline__ void member_reduce_body(const KArgs& a, uint32_t m) {
 }
 
 __global__ __launch_bounds__(kBlock) void param_reduce_kernel(KArgs a) { member_reduce_body(a, blockIdx.x); }
+
+template <int KX, bool DIAG, bool FUSED, bool VJP, bool EXTRA>
+__global__ __launch_bounds__(kBlock) void fwd_kernel(KArgs a) {
+  fwd_body<KX, DIAG, FUSED, VJP, EXTRA>(a, blockIdx.x);
+}
 
 // ---------------------------------------------------------------- batches
 // A batch descriptor holds one KArgs per segment and, per launch, a table of
@@ -1035,6 +1038,16 @@ hipError_t ensure_stack(K kernel) {
   return e;
 }
 
+// The stack limit the batch kernels of one level count need.
+template <int KX>
+hipError_t batch_stack() {
+  hipError_t e = ensure_stack(fwd_batch_kernel<KX>);
+  if (e == hipSuccess) e = ensure_stack(fwd_only_batch_kernel<KX>);
+  if (e == hipSuccess) e = ensure_stack(fwd_plain_batch_kernel<KX>);
+  if (e == hipSuccess) e = ensure_stack(fwd_diag_batch_kernel<KX>);
+  return e;
+}
+
 // Largest grid that is co-resident on the current device for a kernel.
 template <typename K>
 int64_t coresident_blocks(K kernel) {
@@ -1045,18 +1058,28 @@ int64_t coresident_blocks(K kernel) {
   return (int64_t)per_cu * ncu;
 }
 
+// One-time set-up of a kernel family: the stack limit its kernels need and the
+// co-resident grid of its fused kernel.  A function-local static, so the first
+// calls from several threads initialise it once (one GPU per process in this
+// library's deployments).
+struct LaunchInit {
+  hipError_t err;
+  int64_t cap;
+};
+
 template <int KX, bool DIAG, bool VJP, bool EXTRA>
 int launch_fwd(const KArgs& a, hipStream_t s) {
-  static int64_t cap = -1;   // one GPU per process in this library's deployments
-  if (cap < 0) {
+  static const LaunchInit init = [] {
     hipError_t se = ensure_stack(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
     if (se == hipSuccess)
       se = VJP    ? ensure_stack(fwd_batch_kernel<KX>)
            : DIAG ? ensure_stack(fwd_diag_batch_kernel<KX>)
            : EXTRA ? ensure_stack(fwd_only_batch_kernel<KX>) : ensure_stack(fwd_plain_batch_kernel<KX>);
-    if (se != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: stack limit: ") + hipGetErrorString(se));
-    cap = coresident_blocks(fwd_kernel<KX, DIAG, true, VJP, EXTRA>);
-  }
+    return LaunchInit{se, se == hipSuccess ? coresident_blocks(fwd_kernel<KX, DIAG, true, VJP, EXTRA>) : 0};
+  }();
+  if (init.err != hipSuccess)
+    return fail(SPEEDY_ERR_HIP, std::string("speedy: stack limit: ") + hipGetErrorString(init.err));
+  const int64_t cap = init.cap;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(s, &cs);
   // Sixteen-level forward + VJP calls always take the batch route: at sixteen
@@ -1128,6 +1151,9 @@ int launch_fwd(const KArgs& a, hipStream_t s) {
     if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: launch failed: ") + hipGetErrorString(e));
     return SPEEDY_OK;
   }
+  // Per-member sums in a second small kernel.  (Summing them in the last block to
+  // finish instead, after an agent-scope fence per block, cost more than this
+  // launch: the fences write back the L2 caches.)
   if (e == hipSuccess && (VJP || a.f.budget)) {
     const int slot = timer_begin(s, 3);
     hipLaunchKernelGGL(param_reduce_kernel, dim3(a.nmember), dim3(kBlock), 0, s, a);
@@ -1215,7 +1241,11 @@ int speedy_workspace_status(void* workspace, size_t workspace_bytes, uint32_t* s
   hipError_t e = hipMemcpyAsync(status, err, sizeof(uint32_t), hipMemcpyDeviceToHost, (hipStream_t)stream);
   if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
   if (e == hipSuccess && *status) {
-    hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, err, (int64_t)1);
+    // a failed wait leaves late blocks' level-set words and the generation out of
+    // step: the whole workspace starts again from zero
+    const int64_t n = (int64_t)(workspace_bytes / 4);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<uint32_t*>(workspace), n);
     e = hipGetLastError();
   }
   if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy: status read failed: ") + hipGetErrorString(e));
@@ -1338,13 +1368,12 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
   const uint32_t* d_blk = reinterpret_cast<const uint32_t*>(d + L.blk);
   const uint32_t* d_ls = reinterpret_cast<const uint32_t*>(d + L.ls);
   const uint32_t* d_red = reinterpret_cast<const uint32_t*>(d + L.red);
-  // One flag per level count: the three forward kernels share this lambda's
-  // instantiation (same pointer types) but not their private segment sizes.
-  static bool stack_ok[3] = {false, false, false};
-  auto launch = [&](auto ls_kernel, auto fwd_only_b, auto fwd_plain_b, auto fwd_kernel_b, int kxi) {
-    if (!stack_ok[kxi] && ensure_stack(fwd_kernel_b) == hipSuccess && ensure_stack(fwd_only_b) == hipSuccess &&
-        ensure_stack(fwd_plain_b) == hipSuccess)
-      stack_ok[kxi] = true;
+  // The stack limit of each level count's batch kernels, raised once (a
+  // function-local static: thread-safe); a failure fails every batch call.
+  static const hipError_t stack_err[3] = {
+      batch_stack<7>(), batch_stack<8>(), batch_stack<16>()};
+  auto launch = [&](auto ls_kernel, auto fwd_only_b, auto fwd_plain_b, auto fwd_kernel_b, int kxi) -> hipError_t {
+    if (stack_err[kxi] != hipSuccess) return stack_err[kxi];
     if (nl) {
       const int slot = timer_begin(s, 0);
       hipLaunchKernelGGL(ls_kernel, dim3(nl), dim3(kBlock), jmax * sizeof(uint32_t), s, d_args, d_ls);
@@ -1368,19 +1397,22 @@ int speedy_fwd_batch(const SpeedySegment* segs, int32_t nseg, const SpeedyVertic
       hipLaunchKernelGGL(param_reduce_batch_kernel, dim3(nr), dim3(kBlock), 0, s, d_args, d_red);
       timer_end(s, slot2);
     }
+    return hipSuccess;
   };
+  hipError_t le;
   switch (vert->kx) {
     case 7:
-      launch(levelset_batch_kernel<7>, fwd_only_batch_kernel<7>, fwd_plain_batch_kernel<7>, fwd_batch_kernel<7>, 0);
+      le = launch(levelset_batch_kernel<7>, fwd_only_batch_kernel<7>, fwd_plain_batch_kernel<7>, fwd_batch_kernel<7>, 0);
       break;
     case 8:
-      launch(levelset_batch_kernel<8>, fwd_only_batch_kernel<8>, fwd_plain_batch_kernel<8>, fwd_batch_kernel<8>, 1);
+      le = launch(levelset_batch_kernel<8>, fwd_only_batch_kernel<8>, fwd_plain_batch_kernel<8>, fwd_batch_kernel<8>, 1);
       break;
     default:
-      launch(levelset_batch_kernel<16>, fwd_only_batch_kernel<16>, fwd_plain_batch_kernel<16>, fwd_batch_kernel<16>,
-             2);
+      le = launch(levelset_batch_kernel<16>, fwd_only_batch_kernel<16>, fwd_plain_batch_kernel<16>,
+                  fwd_batch_kernel<16>, 2);
       break;
   }
+  if (le != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_batch: stack limit: ") + hipGetErrorString(le));
   e = hipGetLastError();
   if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_fwd_batch: launch failed: ") + hipGetErrorString(e));
   return SPEEDY_OK;

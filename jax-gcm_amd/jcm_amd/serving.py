@@ -38,6 +38,7 @@ import numpy as np
 import torch
 
 from . import _native
+from .forcing import FORCING_FIELDS
 from .params import VJP_PARAMS
 from .physics_interface import PhysicsState, PhysicsTendency
 from .speedy_physics import SpeedyPhysics
@@ -45,6 +46,7 @@ from .speedy_physics import SpeedyPhysics
 KIND_FWD = "fwd"
 KIND_VJP = "fwd_vjp"
 _VJP_NAMES = tuple(f"{g}.{n}" for g, n in VJP_PARAMS)
+_STATE_NAMES = ("u_wind", "v_wind", "temperature", "specific_humidity", "geopotential", "normalized_surface_pressure")
 
 
 @dataclass
@@ -74,7 +76,7 @@ class Request:
     def columns(self) -> int:
         kx, ix, il = self.geometry.nodal_shape
         t = self.state.temperature
-        nmember = 1 if t.dim() == 3 else int(np.prod(t.shape[:-3]))
+        nmember = 1 if len(t.shape) == 3 else int(np.prod(t.shape[:-3]))
         return ix * il * nmember
 
     @property
@@ -175,25 +177,44 @@ class PhysicsServer:
         return fp
 
     def _member(self, req: Request):
-        """Input half of the call structs for a member's (state, forcing, geometry, day),
-        built once: members' states stay resident between requests."""
-        key = (id(req.state), id(req.forcing), id(req.geometry), req.date.model_day())
+        """Input half of the call structs for a member's (state, forcing, geometry, day).
+        Built once and cached only when every state field is a contiguous float32
+        tensor on the device: the kernels then read the client's memory at launch, so
+        in-place updates are seen; each hit checks that the state still holds the
+        same tensors at the same addresses.  States with NumPy or host fields, and
+        forcings with tensor climatologies, are rebuilt on every request (their
+        device copies are made per call).  Forcing and geometry NumPy arrays are
+        treated as constants, like the reference's JAX arrays."""
+        st = req.state
+        fields = tuple(getattr(st, n) for n in _STATE_NAMES)
+        key = (id(st), id(req.forcing), id(req.geometry), req.date.model_day())
         hit = self._members.get(key)
-        if hit is not None and not (hit[5][0]() is req.state and hit[5][1]() is req.forcing
-                                    and hit[5][2]() is req.geometry):
-            hit = None   # an id reused by a new object
-        if hit is None:
-            keep: list = []
-            f, t, nm = self.phys._fields(req.state, req.forcing, req.geometry, req.date, keep)
-            kx, ix, il = req.geometry.nodal_shape
-            flags = _native.FLAG_LFLUXLAND if bool(req.forcing.lfluxland) else 0
+        if hit is not None:
+            f, tshape, grid, flags, keep, refs, objs, ptrs = hit
+            if not (refs[0]() is st and refs[1]() is req.forcing and refs[2]() is req.geometry
+                    and all(a is b for a, b in zip(objs, fields))
+                    and all(a.data_ptr() == p for a, p in zip(fields, ptrs))):
+                hit = None   # an id reused by a new object, or a field replaced
+        if hit is not None:
+            return hit
+        keep: list = []
+        f, t, nm = self.phys._fields(st, req.forcing, req.geometry, req.date, keep)
+        kx, ix, il = req.geometry.nodal_shape
+        flags = _native.FLAG_LFLUXLAND if bool(req.forcing.lfluxland) else 0
+        cacheable = all(isinstance(x, torch.Tensor) and x.dtype is torch.float32 and x.is_contiguous()
+                        and x.get_device() == self._dev_index for x in fields)
+        cacheable &= not any(isinstance(getattr(req.forcing, n, None), torch.Tensor)
+                             and getattr(req.forcing, n).dim() > 2 for n in FORCING_FIELDS)
+        hit = (f, tuple(t.shape), (ix, il, nm), flags, keep, None, fields, tuple(x.data_ptr() for x in fields)
+               if cacheable else ())
+        if cacheable:
             # weak references: the entry (and the device tensors it keeps) goes
             # when the client drops the state, so states that are new with every
             # request are not held here
             members = self._members
             drop = lambda _ref, k=key, d=members: d.pop(k, None)
-            refs = (weakref.ref(req.state, drop), weakref.ref(req.forcing), weakref.ref(req.geometry))
-            hit = (f, tuple(t.shape), (ix, il, nm), flags, keep, refs)
+            refs = (weakref.ref(st, drop), weakref.ref(req.forcing), weakref.ref(req.geometry))
+            hit = hit[:5] + (refs,) + hit[6:]
             if len(members) > 4096:
                 members.clear()
             members[key] = hit
@@ -201,7 +222,7 @@ class PhysicsServer:
 
     def _prepare(self, req: Request, seg, stream):
         """Fill one batch segment for req; allocates its outputs in one block."""
-        f_in, tshape, (ix, il, nm), flags, _, _ = self._member(req)
+        f_in, tshape, (ix, il, nm), flags, member_keep = self._member(req)[:5]
         seg.fields = f_in
         f = seg.fields                     # a view of the segment's memory
         n3 = math.prod(tshape)
@@ -227,7 +248,7 @@ class PhysicsServer:
             adj.g_ps, adj.g_params = base + 8 * step, base + 8 * step + 4 * n2
             req.state_grads = PhysicsState(views[4], views[5], views[6], views[7], None, g_ps)
             req.param_grads = dict(zip(_VJP_NAMES, g_par.unbind(-1)))
-        return (ix, il, nm), flags, buf
+        return (ix, il, nm), flags, buf, member_keep
 
     def _cotangent_ptrs(self, ct: PhysicsTendency, tshape) -> tuple:
         """Device addresses of a cotangent's four fields, after checking them.
@@ -241,8 +262,16 @@ class PhysicsServer:
 
     # ------------------------------------------------------------ loop
     def submit(self, req: Request):
+        """Queue a request.  Everything that can be wrong with it is checked here,
+        before it is queued, so a request is never dropped after admission."""
         if self.footprint(req) > self.hbm_budget:
             raise _native.SpeedyError(f"request {req.rid} needs more HBM than the whole budget")
+        kx, ix, il = req.geometry.nodal_shape
+        shape = tuple(req.state.temperature.shape)
+        if shape[-3:] != (kx, ix, il):
+            raise ValueError(f"request {req.rid}: state shape {shape} does not match geometry {(kx, ix, il)}")
+        if req.cotangent is not None:
+            self._cotangent_ptrs(req.cotangent, shape)
         self.queue.append(req)
 
     def _free_slot(self) -> Optional[_Slot]:
@@ -261,6 +290,8 @@ class PhysicsServer:
                 break
             if batch and cols + req.columns > self.max_batch_columns:
                 break
+            if batch and req.geometry.nodal_shape[0] != batch[0].geometry.nodal_shape[0]:
+                break   # a batch shares one vertical grid; this request opens the next batch
             self.queue.popleft()
             self.in_flight_bytes += fp
             self.peak_bytes = max(self.peak_bytes, self.in_flight_bytes)
@@ -278,8 +309,9 @@ class PhysicsServer:
         self._wait_client(slot)
         for i, req in enumerate(batch):
             seg = segs[i]
-            (ix, il, nm), flags, buf = self._prepare(req, seg, slot.stream)
+            (ix, il, nm), flags, buf, member_keep = self._prepare(req, seg, slot.stream)
             keep.append(buf)
+            keep.append(member_keep)   # device copies of a NumPy state live until the batch completes
             ws = self._workspace((ix, il, nm), slot.stream)
             req._ws = ((ix, il, nm), ws)
             seg.date = req.date.to_abi()
@@ -291,9 +323,6 @@ class PhysicsServer:
         if hit is None or hit[0] is not batch[0].geometry:
             hit = self._verts[gkey] = (batch[0].geometry, batch[0].geometry.to_abi())
         vert = hit[1]
-        for req in batch[1:]:
-            if req.geometry.nodal_shape[0] != batch[0].geometry.nodal_shape[0]:
-                raise _native.SpeedyError("a batch shares one vertical grid")
         nbytes = lib.speedy_batch_desc_bytes(segs, len(batch))
         if nbytes == 0:
             raise _native.SpeedyError("speedy_batch_desc_bytes failed: " + lib.speedy_last_error().decode())

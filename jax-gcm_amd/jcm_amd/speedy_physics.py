@@ -145,15 +145,19 @@ class SpeedyPhysics(Physics):
 
     def __init__(self, parameters: Parameters | None = None, checkpoint_terms: bool = True,
                  with_diagnostics: bool = False, device: str | torch.device | None = None,
-                 plain_launch: bool = False):
+                 plain_launch: bool = False, check_barrier: bool = False):
         """plain_launch: uncaptured single-launch calls use a plain launch instead of a
         cooperative one (about 10 us less per call at T85).  Only for a process that
         has the GPU to itself: the grid-wide level-set exchange then relies on all
-        workgroups being resident together, and barrier_status() reports a call
-        whose exchange gave up (its results are invalid)."""
+        workgroups being resident together.  barrier_status() reports a call whose
+        exchange gave up (its results are invalid) and resets the workspace.
+        check_barrier: after every call, synchronize and raise SpeedyError if the
+        call's level-set exchange gave up, resetting the workspace (costs a
+        synchronization per call)."""
         self.parameters = parameters if parameters is not None else Parameters.default()
         self.with_diagnostics = with_diagnostics
         self._launch_flags = _native.FLAG_PLAIN_LAUNCH if plain_launch else 0
+        self.check_barrier = check_barrier
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device()) \
             if torch.cuda.is_available() else None
         self._lib = _native.load()
@@ -193,16 +197,23 @@ class SpeedyPhysics(Physics):
             hit = self._verts[id(geometry)] = (geometry, key, geometry.to_abi())
         return hit[2]
 
-    def workspace(self, nlon: int, nlat: int, nmember: int) -> torch.Tensor:
-        key = (nlon, nlat, nmember)
+    def workspace(self, nlon: int, nlat: int, nmember: int, stream=None) -> torch.Tensor:
+        """The call workspace of this grid on `stream` (the current stream when None).
+        Workspaces are per (grid, stream): calls in flight on different streams never
+        share one (speedy_abi.h: concurrent calls need distinct workspaces)."""
+        s = stream if stream is not None else self._stream()
+        key = (nlon, nlat, nmember, s.value)
         ws = self._workspaces.get(key)
         if ws is None:
             nbytes = self._lib.speedy_workspace_bytes(nlon, nlat, nmember)
             if nbytes == 0:
                 raise _native.SpeedyError(f"grid {nlon}x{nlat}x{nmember} is not supported")
+            if torch.cuda.is_current_stream_capturing():
+                # its initialising fill would be captured and replayed with every call
+                raise _native.SpeedyError("a new workspace is needed during graph capture: call once on this "
+                                          "stream and grid outside capture first")
             ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
-            _native.check(self._lib.speedy_workspace_init(ws.data_ptr(), nbytes, self._stream()),
-                          "speedy_workspace_init")
+            _native.check(self._lib.speedy_workspace_init(ws.data_ptr(), nbytes, s), "speedy_workspace_init")
             self._workspaces[key] = ws
         return ws
 
@@ -331,13 +342,16 @@ class SpeedyPhysics(Physics):
         if budgets:
             self.last_budgets = torch.empty((nmember, _native.NBUDGET), dtype=torch.float64, device=self.device)
             f.budget = self.last_budgets.data_ptr()
-        ws = self.workspace(ix, il, nmember)
+        stream = self._stream()
+        ws = self.workspace(ix, il, nmember, stream)
         flags = ((_native.FLAG_LFLUXLAND if bool(forcing.lfluxland) else 0) | (VERIFY_FLAGS if verify else 0)
                  | self._launch_flags)
         rc = self._lib.speedy_fwd(ctypes.byref(f), ctypes.byref(self._vertical(geometry)), ctypes.byref(self._params()),
                                   ctypes.byref(date.to_abi()), ix, il, nmember, flags, term_mask, level_set,
-                                  ws.data_ptr(), ws.numel(), self._stream())
+                                  ws.data_ptr(), ws.numel(), stream)
         _native.check(rc, "speedy_fwd")
+        if self.check_barrier:
+            self._raise_on_barrier(ix, il, nmember, stream)
         data = PhysicsData(diag, _native.diag_names(kx), kx, (ix, il)) if diag is not None else None
         return PhysicsTendency(*out), data
 
@@ -392,15 +406,27 @@ class SpeedyPhysics(Physics):
         keep: list = []
         f, adj, outputs, (ix, il, nmember), flags = self.prepare_vjp(state, forcing, geometry, date, cotangent, keep)
         flags |= (VERIFY_FLAGS if verify else 0) | self._launch_flags
-        ws = self.workspace(ix, il, nmember)
+        stream = self._stream()
+        ws = self.workspace(ix, il, nmember, stream)
         rc = self._lib.speedy_fwd_vjp(ctypes.byref(f), ctypes.byref(adj), ctypes.byref(self._vertical(geometry)),
                                       ctypes.byref(self._params()), ctypes.byref(date.to_abi()), ix, il, nmember,
-                                      flags, level_set, ws.data_ptr(), ws.numel(), self._stream())
+                                      flags, level_set, ws.data_ptr(), ws.numel(), stream)
         _native.check(rc, "speedy_fwd_vjp")
+        if self.check_barrier:
+            self._raise_on_barrier(ix, il, nmember, stream)
         return outputs["tendency"], outputs["state_grads"], outputs["param_grads"]
 
+    def _raise_on_barrier(self, nlon, nlat, nmember, stream):
+        ws = self.workspace(nlon, nlat, nmember, stream)
+        st = ctypes.c_uint32(0)
+        _native.check(self._lib.speedy_workspace_status(ws.data_ptr(), ws.numel(), ctypes.byref(st), stream),
+                      "speedy_workspace_status")
+        if st.value:
+            raise _native.SpeedyError("the call's level-set exchange gave up (not all workgroups were resident); "
+                                      "its results are invalid and the workspace has been reset")
+
     def level_sets(self, nlon: int, nlat: int, nmember: int = 1) -> list:
-        """Q1 level set of each member in the last call on this grid."""
+        """Q1 level set of each member in the last call on this grid and the current stream."""
         ws = self.workspace(nlon, nlat, nmember)
         buf = (ctypes.c_uint32 * nmember)()
         _native.check(self._lib.speedy_read_level_sets(ws.data_ptr(), ws.numel(), nmember, buf, self._stream()),
@@ -408,8 +434,10 @@ class SpeedyPhysics(Physics):
         return list(buf)
 
     def barrier_status(self, nlon: int, nlat: int, nmember: int = 1) -> int:
-        """Nonzero if a call on this grid since the last query gave up waiting for the
-        level set (its results are invalid); reading clears the flag."""
+        # (a set flag also re-zeroes the workspace: see speedy_workspace_status)
+        """Nonzero if a call on this grid and the current stream since the last query
+        gave up waiting for the level set (its results are invalid); reading clears
+        the flag."""
         ws = self.workspace(nlon, nlat, nmember)
         st = ctypes.c_uint32(0)
         _native.check(self._lib.speedy_workspace_status(ws.data_ptr(), ws.numel(), ctypes.byref(st), self._stream()),

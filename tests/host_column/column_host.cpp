@@ -1,7 +1,9 @@
 // column_host.cpp -- TEST HARNESS ONLY: the kernel's column routine
 // (jax-gcm_amd/csrc/speedy_column.hpp) compiled for the host with g++, so the
 // CPU test suite can check the device code's logic against the oracle without
-// a GPU.  It is never loaded by the library or the product path.
+// a GPU.  Built twice: without OpenMP for the tests, and with -fopenmp (columns
+// split over the host's cores) as the CPU baseline that bench.py times.  It is
+// never loaded by the library or the product path.
 #include <stdint.h>
 #include <string.h>
 
@@ -37,6 +39,7 @@ static void run(const R* const* in, R* const* out, const SpeedyVertical* v, cons
     col.stratz = lat[4][j]; col.sqcoa = lat[5][j];
   };
   uint32_t S = 0;
+#pragma omp parallel for reduction(| : S) schedule(static)
   for (int64_t c = 0; c < ncol; ++c) {
     ColumnIn<R, KX> col;
     load(c, col);
@@ -46,6 +49,7 @@ static void run(const R* const* in, R* const* out, const SpeedyVertical* v, cons
   }
   *s_out = S;
   if (level_set >= 0) S = (uint32_t)level_set;
+#pragma omp parallel for schedule(static)
   for (int64_t c = 0; c < ncol; ++c) {
     ColumnIn<R, KX> col;
     load(c, col);
@@ -99,6 +103,7 @@ static void run_vjp(const R* const* in, const R* const* ct, R* const* out, R* co
     col.fsol = z[0]; col.ozupp = z[1]; col.ozone = z[2]; col.zenit = z[3]; col.stratz = z[4]; col.sqcoa = z[5];
   };
   uint32_t S = 0;
+#pragma omp parallel for reduction(| : S) schedule(static)
   for (int64_t c = 0; c < ncol; ++c) {
     ColumnIn<R, KX> col;
     load(c, col);
@@ -107,6 +112,7 @@ static void run_vjp(const R* const* in, const R* const* ct, R* const* out, R* co
     S |= 1u << icl;
   }
   if (level_set >= 0) S = (uint32_t)level_set;
+#pragma omp parallel for schedule(static)
   for (int64_t c = 0; c < ncol; ++c) {
     ColumnIn<R, KX> col;
     load(c, col);

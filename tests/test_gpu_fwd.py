@@ -674,3 +674,71 @@ def test_diag_accumulate_tails_and_alignment(n, offset):
         a[offset:offset + n] = a[offset:offset + n] + d[offset:offset + n] / np.float32(7.0)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(_np(ad), a)
+
+
+@pytest.mark.parametrize("vjp", [False, True])
+def test_failed_level_set_wait_is_reported_and_reset(vjp):
+    """A fused call whose level-set wait gave up must not pass unnoticed.  The
+    condition is forced from the host, as a timed-out wait leaves it: the
+    workspace's error word is set and its level-set words are left stale (no
+    kernel waits or stalls).  barrier_status() reports the flag and resets the
+    whole workspace, after which a call equals a clean call bit for bit; with
+    check_barrier=True the call itself raises."""
+    import torch
+    from jcm_amd import _native
+    from jcm_amd.physics_interface import PhysicsTendency
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    case = synthetic.make_case(64, 32, 8, seed=3, land=True)
+    kx, ix, il = case[2].nodal_shape
+    phys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2))
+    cot = PhysicsTendency(*[torch.randn((kx, ix, il), device=phys.device) for _ in range(4)])
+    run = (lambda p: p.compute_tendencies_vjp(*case, cot)) if vjp else (lambda p: p.compute_tendencies(*case))
+    clean = run(phys)
+    torch.cuda.synchronize()
+    ws = phys.workspace(ix, il, 1)
+    ws.view(torch.int32)[1] = 1            # the error word, as a timed-out wait sets it
+    assert phys.barrier_status(ix, il, 1) == 1
+    assert int(ws.view(torch.int32).abs().max()) == 0, "the workspace was not reset"
+    again = run(phys)
+    torch.cuda.synchronize()
+    for f in FIELDS:
+        assert torch.equal(getattr(again[0], f), getattr(clean[0], f)), f
+    assert phys.barrier_status(ix, il, 1) == 0
+    checked = SpeedyPhysics(parameters=phys.parameters, check_barrier=True)
+    run(checked)
+    checked.workspace(ix, il, 1).view(torch.int32)[1] = 1
+    with pytest.raises(_native.SpeedyError, match="gave up"):
+        run(checked)
+    again2 = run(checked)   # the workspace was reset: clean again
+    torch.cuda.synchronize()
+    assert torch.equal(again2[0].temperature, clean[0].temperature)
+
+
+@pytest.mark.parametrize("kx", [7, 8, 16])
+@pytest.mark.parametrize("variant", ["plain", "diag", "verify_budgets", "diag_budgets"])
+def test_fwd_fused_equals_batch_route(kx, variant):
+    """Every single-launch forward kernel (plain, with diagnostics, with the verify
+    steps and float64 budgets) gives, bit for bit, what the batch route gives with
+    the same level set: the same column code, whichever kernel runs it.  Three
+    calls in a row on one workspace (no state carried between calls)."""
+    import torch
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    case = synthetic.make_case(64, 32, kx, seed=1, land=True)
+    diag = variant.startswith("diag")
+    budgets = variant.endswith("budgets")
+    verify = variant == "verify_budgets"
+    phys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2), with_diagnostics=diag)
+    for _ in range(3):
+        t_f, d_f = phys.compute_tendencies(*case, verify=verify, budgets=budgets)
+        b_f = phys.last_budgets.clone() if budgets else None
+        S = phys.level_sets(64, 32, 1)[0]
+        t_b, d_b = phys.compute_tendencies(*case, verify=verify, budgets=budgets, level_set=S)
+        b_b = phys.last_budgets.clone() if budgets else None
+        torch.cuda.synchronize()
+        for f in FIELDS:
+            bad = int((getattr(t_f, f) != getattr(t_b, f)).sum())
+            assert bad == 0, f"kx{kx} {variant} {f}: {bad} entries differ from the batch route"
+        if diag:
+            assert torch.equal(d_f._diag, d_b._diag), f"kx{kx} {variant}: diagnostics differ"
+        if budgets:
+            assert torch.equal(b_f, b_b), f"kx{kx} {variant}: budgets {b_f} vs {b_b}"

@@ -210,3 +210,88 @@ def test_admission_respects_hbm_budget_and_overlaps():
     assert st["p99_ms"] >= st["p50_ms"] > 0
     late = [r for r in reqs if r.batch > 0 and r.arrival > min(x.t_admit for x in reqs)]
     assert late, "no request was admitted after the first batch had started"
+
+
+def test_per_request_two_states_same_grid_two_streams():
+    """batching=False with two slot streams: two requests on the same grid with
+    different states are in flight together; each equals the same request alone,
+    bit for bit (each stream has its own workspace: level-set words, generation,
+    parameter partials)."""
+    import torch
+    from jcm_amd.serving import PhysicsServer, Request
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2))
+    reqs = []
+    for i in range(6):
+        st, fo, ge, da = _device_case(256, 128, 90 + i)
+        if i % 2:   # wetter states: other level sets than the dry ones
+            st = st.copy(specific_humidity=st.specific_humidity * 1.12)
+        cot = PhysicsTendency(*[torch.randn(st.temperature.shape, device=st.temperature.device) for _ in range(4)]) \
+            if i % 3 == 2 else None
+        reqs.append(Request(state=st, forcing=fo, geometry=ge, date=da, cotangent=cot, rid=i))
+    server = PhysicsServer(phys, n_streams=2, batching=False)
+    server.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    alone = SpeedyPhysics(parameters=phys.parameters)
+    for r in reqs:
+        if r.cotangent is None:
+            t, _ = alone.compute_tendencies(r.state, r.forcing, r.geometry, r.date)
+        else:
+            t, sg, pg = alone.compute_tendencies_vjp(r.state, r.forcing, r.geometry, r.date, r.cotangent)
+        torch.cuda.synchronize()
+        for f in ("u_wind", "v_wind", "temperature", "specific_humidity"):
+            assert torch.equal(getattr(r.tendency, f), getattr(t, f)), f"req {r.rid} {f}"
+        if r.cotangent is not None:
+            for f in ("temperature", "specific_humidity", "normalized_surface_pressure"):
+                assert torch.equal(getattr(r.state_grads, f), getattr(sg, f)), f"req {r.rid} grad {f}"
+            for k in pg:
+                assert torch.equal(r.param_grads[k], pg[k]), f"req {r.rid} {k}"
+
+
+@pytest.mark.parametrize("batching", [True, False])
+def test_numpy_state_edited_in_place_between_requests(batching):
+    """A NumPy state is copied to the device for each request (it is not cached),
+    so an in-place edit between two requests is seen by the second."""
+    import torch
+    from jcm_amd.serving import PhysicsServer, Request
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    st, fo, ge, da = synthetic.make_case(64, 32, 8, seed=12, land=True)
+    server = PhysicsServer(phys, n_streams=1, batching=batching)
+    r0 = Request(state=st, forcing=fo, geometry=ge, date=da, rid=0)
+    server.run([r0], realtime=False)
+    torch.cuda.synchronize()
+    st.temperature += np.float32(2.0)                    # in place, same object
+    r1 = Request(state=st, forcing=fo, geometry=ge, date=da, rid=1)
+    server.run([r1], realtime=False)
+    torch.cuda.synchronize()
+    want, _ = phys.compute_tendencies(st, fo, ge, da)
+    torch.cuda.synchronize()
+    assert torch.equal(r1.tendency.temperature, want.temperature)
+    assert not torch.equal(r0.tendency.temperature, r1.tendency.temperature)
+
+
+def test_mixed_level_counts_in_one_stream():
+    """Requests with 8 and 16 levels interleaved: each batch holds one vertical
+    grid, every request is served, and each result equals the request alone."""
+    import torch
+    from jcm_amd.serving import PhysicsServer, Request
+    from jcm_amd.speedy_physics import SpeedyPhysics
+    phys = SpeedyPhysics()
+    reqs = []
+    for i in range(8):
+        kx = 8 if i % 2 == 0 else 16
+        st, fo, ge, da = synthetic.make_case(64, 32, kx, seed=60 + i, land=True)
+        dev = torch.device("cuda", 0)
+        st = st.copy(**{k: torch.as_tensor(np.ascontiguousarray(getattr(st, k)), device=dev)
+                        for k in H.STATE_FIELDS})
+        reqs.append(Request(state=st, forcing=fo, geometry=ge, date=da, rid=i))
+    server = PhysicsServer(phys, n_streams=2)
+    done = server.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    assert len(done) == len(reqs)
+    assert server.in_flight_bytes == 0
+    for r in reqs:
+        t, _ = phys.compute_tendencies(r.state, r.forcing, r.geometry, r.date)
+        torch.cuda.synchronize()
+        assert torch.equal(r.tendency.temperature, t.temperature), r.rid

@@ -319,3 +319,24 @@ def test_vjp_batch_route_other_level_counts(phys, kx):
         _, g64 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m])
         _, g32 = H.run_host_vjp((sm, fo, ge, da), params, [c[m] for c in cot], level_set=S[m], dtype=np.float32)
         _check_against_host({k: v[m] for k, v in g.items()}, prm[m], g64, f"kx {kx} member {m}", g32)
+
+
+@pytest.mark.parametrize("name", ["T21_wet", "T21_wet_kx7", "T21_land"])
+def test_vjp_fused_equals_batch_route(phys, name):
+    """The single-launch forward + VJP (its level-set exchange and its own parameter
+    sums) and the batch route given the same level set run the same column code:
+    tendencies, state gradients and parameter gradients are bit-identical, over
+    three repeated single-launch calls (no race, no state carried between calls)."""
+    params = synthetic.draw_vjp_parameters(2)
+    case = make(name)
+    kx, ix, il = case[2].nodal_shape
+    cot = _cot((kx, ix, il), 5)
+    runs = [_gpu_vjp(phys, case, params, cot) for _ in range(3)]
+    S = phys.level_sets(ix, il, 1)[0]
+    tb, gb, pb = _gpu_vjp(phys, case, params, cot, level_set=S)
+    for tend, g, prm in runs:
+        for f in FIELDS:
+            assert np.array_equal(_np(getattr(tend, f)), _np(getattr(tb, f))), f
+        for k in g:
+            assert np.array_equal(g[k], gb[k]), k
+        assert np.array_equal(prm, pb)
