@@ -170,7 +170,7 @@ class GraphStep:
     def __init__(self, fn, calls, dev):
         import torch
         self.calls = calls
-        stream = torch.cuda.Stream(device=dev)
+        self.stream = stream = torch.cuda.Stream(device=dev)
         stream.wait_stream(torch.cuda.current_stream(dev))
         with torch.cuda.stream(stream):
             fn()                                  # warm the side stream
@@ -183,15 +183,10 @@ class GraphStep:
         self.graph.replay()
         torch.cuda.synchronize()
 
-    def time(self, reps, warm_ms=20.0):
-        """ms per call over reps replays, HIP events on the replaying stream.  Untimed
-        replays (about warm_ms of device work) are queued right ahead of the first
-        event: the GPU's clocks are up and the stream is busy when the timed replays
-        start, so a short measurement (one replay of a 20-call graph) sees the same
-        per-call time as a long one instead of the clock ramp after setup."""
+    def queue(self, warm_ms):
+        """Queue untimed replays worth about warm_ms of device work and return at once:
+        the GPU's clocks are up and the stream is busy when the next work starts."""
         import torch
-        t0 = torch.cuda.Event(enable_timing=True)
-        t1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         w0 = time.perf_counter()
         self.graph.replay()
@@ -199,6 +194,16 @@ class GraphStep:
         one = max(time.perf_counter() - w0, 1e-6)
         for _ in range(max(1, int(warm_ms * 1e-3 / one))):
             self.graph.replay()
+
+    def time(self, reps, warm_ms=20.0):
+        """ms per call over reps replays, HIP events on the replaying stream.  Untimed
+        replays (about warm_ms of device work) are queued right ahead of the first
+        event, so a short measurement (one replay of a 20-call graph) sees the same
+        per-call time as a long one instead of the clock ramp after setup."""
+        import torch
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        self.queue(warm_ms)
         t0.record()
         for _ in range(reps):
             self.graph.replay()
@@ -413,10 +418,18 @@ def main():
     launches = {k: v[1] for k, v in tm.items() if v[1]}
 
     # per-kernel device time of the forward + VJP call (HIP events around each of
-    # its two kernels, plain calls): the VJP kernel without the parameter sums
-    lib.speedy_timer_enable(64)
-    for _ in range(20):
-        vjp()
+    # its two kernels): the VJP kernel without the parameter sums.  The calls use
+    # plain launches, as the graph replays do (a cooperative launch waits for an
+    # idle device), and queue behind ~20 ms of graph replays on the same stream, so
+    # each event pair brackets its kernel and not the host's launch gaps.
+    p_phys = SpeedyPhysics(device=dev, plain_launch=True)
+    p_vphys = SpeedyPhysics(parameters=vphys.parameters, device=dev, plain_launch=True)
+    with torch.cuda.stream(g_vjp.stream):
+        p_vphys.compute_tendencies_vjp(*case, cot)
+        g_vjp.queue(20.0)
+        lib.speedy_timer_enable(64)
+        for _ in range(20):
+            p_vphys.compute_tendencies_vjp(*case, cot)
     torch.cuda.synchronize()
     tm_v = _native.timer_read()
     lib.speedy_timer_disable()
@@ -460,8 +473,6 @@ def main():
     pc_f = per_call_ms(fwd, args.per_call_steps)
     pc_v = per_call_ms(vjp, args.per_call_steps)
     # the same calls with plain instead of cooperative launches (plain_launch=True)
-    p_phys = SpeedyPhysics(device=dev, plain_launch=True)
-    p_vphys = SpeedyPhysics(parameters=vphys.parameters, device=dev, plain_launch=True)
     pc_pf = per_call_ms(lambda: p_phys.compute_tendencies(*case), args.per_call_steps)
     pc_pv = per_call_ms(lambda: p_vphys.compute_tendencies_vjp(*case, cot), args.per_call_steps)
     out["per_call"] = {"fwd_columns_per_s": ncol / (pc_f * 1e-3), "fwd_ms": pc_f,

@@ -1,0 +1,715 @@
+// Dynamics <-> physics transforms for gfx950: spherical-harmonic analysis and
+// synthesis on a Gaussian grid as batched fp32 MFMA GEMMs, with the unit
+// conversions, the modal vorticity/divergence <-> wind operators and the
+// hydrostatic geopotential fused around them.
+//
+// Reference behaviour: jcm/physics_interface.py:196-303 over dinosaur's
+// RealSphericalHarmonics (Grid.T<N>, jcm/utils.py:28-54).
+//
+// Data flow of one conversion (B stacked fields, Mm = 2*n_m-1 modes):
+//   to_nodal:  modal [B][Mm][L] --Legendre--> Y [B][Mm][nlat] --Fourier--> nodal [B][nlon][nlat]
+//   to_modal:  nodal [B][nlon][nlat] --Fourier--> Y [B][Mm][nlat] --Legendre--> modal [B][Mm][L]
+// Each arrow is one launch of sht_gemm_kernel<STAGE>: a 64x64 output tile per
+// workgroup of four waves, each wave 32x32 as 2x2 v_mfma_f32_16x16x4f32, K in
+// steps of 16 staged through LDS.  The Fourier stages batch over fields (the
+// basis F is shared), the Legendre stages batch over modes (the basis P[m] is
+// shared by every field).  Per-field pre/post operations (1/cos(lat), unit
+// scales, reference temperature, log/exp of surface pressure) ride in the
+// Fourier stages' tile loads and stores; the l +- 1 modal operators are one
+// elementwise kernel on either side.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "speedy_sht.h"
+
+struct SpeedyShtPlan {
+  int32_t n_m, n_l, nlon, nlat, mm;   // mm = 2*n_m-1 modes
+  double radius;
+  int device;
+  float* F;     // [nlon][mm]
+  float* P;     // [n_m][nlat][n_l]
+  float* w;     // [nlat] Gauss weights * 2 pi / nlon
+  float* sec;   // [nlat] 1 / cos(lat)
+};
+
+namespace sht {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+constexpr int kThreads = 256, kBM = 64, kBN = 64, kBK = 16, kPad = 4;
+constexpr int kMaxSeg = 8;
+
+enum Stage { kFourierAnalysis = 0, kLegendreAnalysis = 1, kLegendreSynthesis = 2, kFourierSynthesis = 3 };
+enum : int32_t { kOpLog = 1, kOpExp = 2, kOpSecLat = 4, kOpSubLevel = 8, kOpAddLevel = 16 };
+
+// A run of stacked fields that share a source (Fourier analysis) or a
+// destination (Fourier synthesis) and one operation.
+struct Seg {
+  const float* src;
+  float* dst;
+  const float* level;   // per-field constant, index (f - first) % period, or null
+  int32_t first, count, period, op;
+  float mul;
+};
+struct SegTable {
+  Seg s[kMaxSeg];
+  int32_t n;
+};
+
+struct StageArgs {
+  int32_t nlon, nlat, mm, nl, nfield;
+  const float* F;
+  const float* P;
+  const float* w;
+  const float* sec;
+  const float* in;    // LA: Y, LS: modal X, FS: Y
+  float* out;         // FA: Y, LA: modal, LS: Y
+  SegTable seg;       // FA: sources, FS: destinations (sorted by first)
+};
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int find_seg(const SegTable& t, int f) {
+  int k = 0;
+  for (int i = 1; i < t.n; ++i)
+    if (f >= t.s[i].first) k = i;
+  return k;
+}
+
+// C[M x N] = A[M x K] * B[K x N] for one batch entry (blockIdx.y); element
+// (r, c) of X at X[r * x_r + c * x_c].
+template <int STAGE>
+__global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
+  const int b = blockIdx.y;
+  int M, N, K;
+  const float* A;
+  const float* B;
+  float* C;
+  long a_m, a_k, b_k, b_n, c_m;
+  const int mode_m = (b + 1) >> 1;    // Legendre stages: order m of mode b
+  Seg sg{};
+  float lvl = 0.0f;
+  if (STAGE == kFourierAnalysis) {       // Y_b[mm x nlat] = F^T[mm x nlon] * x_b[nlon x nlat]
+    sg = a.seg.s[find_seg(a.seg, b)];
+    M = a.mm; N = a.nlat; K = a.nlon;
+    A = a.F; a_m = 1; a_k = a.mm;
+    B = sg.src + (size_t)(b - sg.first) * a.nlon * a.nlat; b_k = a.nlat; b_n = 1;
+    C = a.out + (size_t)b * a.mm * a.nlat; c_m = a.nlat;
+    if (sg.level) lvl = sg.level[(b - sg.first) % sg.period];
+  } else if (STAGE == kLegendreAnalysis) {   // Z[:, b, :] [nfield x L] = Y[:, b, :] [nfield x nlat] * P_m [nlat x L]
+    M = a.nfield; N = a.nl; K = a.nlat;
+    A = a.in + (size_t)b * a.nlat; a_m = (long)a.mm * a.nlat; a_k = 1;
+    B = a.P + (size_t)mode_m * a.nlat * a.nl; b_k = a.nl; b_n = 1;
+    C = a.out + (size_t)b * a.nl; c_m = (long)a.mm * a.nl;
+  } else if (STAGE == kLegendreSynthesis) {  // Y[:, b, :] [nfield x nlat] = X[:, b, :] [nfield x L] * P_m^T [L x nlat]
+    M = a.nfield; N = a.nlat; K = a.nl;
+    A = a.in + (size_t)b * a.nl; a_m = (long)a.mm * a.nl; a_k = 1;
+    B = a.P + (size_t)mode_m * a.nlat * a.nl; b_k = 1; b_n = a.nl;
+    C = a.out + (size_t)b * a.nlat; c_m = (long)a.mm * a.nlat;
+  } else {                                   // out_b [nlon x nlat] = F [nlon x mm] * Y_b [mm x nlat]
+    sg = a.seg.s[find_seg(a.seg, b)];
+    M = a.nlon; N = a.nlat; K = a.mm;
+    A = a.F; a_m = a.mm; a_k = 1;
+    B = a.in + (size_t)b * a.mm * a.nlat; b_k = a.nlat; b_n = 1;
+    C = sg.dst + (size_t)(b - sg.first) * a.nlon * a.nlat; c_m = a.nlat;
+    if (sg.level) lvl = sg.level[(b - sg.first) % sg.period];
+  }
+  constexpr bool kAKContig = STAGE != kFourierAnalysis;
+  constexpr bool kBNContig = STAGE != kLegendreSynthesis;
+
+  const int tiles_n = (N + kBN - 1) / kBN;
+  const int tiles_m = (M + kBM - 1) / kBM;
+  if ((int)blockIdx.x >= tiles_m * tiles_n) return;    // uniform per workgroup
+  const int m0 = (blockIdx.x / tiles_n) * kBM, n0 = (blockIdx.x % tiles_n) * kBN;
+
+  __shared__ float As[kBK][kBM + kPad];
+  __shared__ float Bs[kBK][kBN + kPad];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  for (int k0 = 0; k0 < K; k0 += kBK) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = t + kThreads * r;
+      const int mm = kAKContig ? (e >> 4) : (e & 63);
+      const int kk = kAKContig ? (e & 15) : (e >> 6);
+      const int gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < M && gk < K) ? A[gm * a_m + gk * a_k] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = t + kThreads * r;
+      const int nn = kBNContig ? (e & 63) : (e >> 4);
+      const int kk = kBNContig ? (e >> 6) : (e & 15);
+      const int gk = k0 + kk, gn = n0 + nn;
+      float v = 0.0f;
+      if (gk < K && gn < N) {
+        v = B[gk * b_k + gn * b_n];
+        if (STAGE == kFourierAnalysis) {
+          if (sg.op & kOpLog) v = logf(v);
+          if (sg.op & kOpSubLevel) v -= lvl;
+          v *= sg.mul * a.w[gn];
+          if (sg.op & kOpSecLat) v *= a.sec[gn];
+        }
+      }
+      Bs[kk][nn] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 4) {
+      float av[2], bv[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) av[i] = As[kk + (lane >> 4)][wm + 16 * i + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bv[j] = Bs[kk + (lane >> 4)][wn + 16 * j + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // lane holds C[16 i + 4 (lane >> 4) + r][16 j + (lane & 15)] of its wave's 32 x 32
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gn = n0 + wn + 16 * j + (lane & 15);
+      if (gn >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        if (gm >= M) continue;
+        float v = acc[i][j][r];
+        if (STAGE == kFourierSynthesis) {
+          if (sg.op & kOpSecLat) v *= a.sec[gn];
+          v = v * sg.mul;
+          if (sg.op & kOpAddLevel) v += lvl;
+          if (sg.op & kOpExp) v = expf(v);
+        }
+        C[gm * c_m + gn] = v;
+      }
+    }
+}
+
+// eps(m, l) = sqrt((l^2 - m^2) / (4 l^2 - 1)) for m <= l, l >= 1; else 0:
+// x P_l^m = eps(m, l+1) P_{l+1}^m + eps(m, l) P_{l-1}^m (unit-norm functions).
+__device__ __forceinline__ float eps_ml(int m, int l) {
+  if (l <= 0 || m > l) return 0.0f;
+  const float fl = (float)l, fm = (float)m;
+  return sqrtf((fl * fl - fm * fm) / (4.0f * fl * fl - 1.0f));
+}
+
+struct ModalArgs {
+  int32_t mm, nl, kx;
+  float inv_radius, radius2;
+  const float* vor;     // [kx][mm][nl]
+  const float* div;
+  const float* copy_src[4];   // modal fields copied behind the winds (null: zeros)
+  int32_t copy_count[4];
+  float* out;           // contiguous [nfield][mm][nl]
+  float* vor_out;       // post: destinations of vorticity / divergence
+  float* div_out;
+  float* copy_dst[4];
+};
+
+__device__ __forceinline__ float d_dlon(const float* x, int mode, int m, int l, int nl) {
+  if (mode == 0) return 0.0f;
+  return (mode & 1) ? (float)m * x[(mode + 1) * nl + l] : -(float)m * x[(mode - 1) * nl + l];
+}
+
+// Modal (psi, chi) -> (u cos(lat), v cos(lat)) for every level, then copies of
+// the remaining modal inputs, into one contiguous stack (vor_div_to_uv_nodal's
+// modal half and compute_diagnostic_state's inputs).
+__global__ __launch_bounds__(kThreads) void modal_winds_kernel(ModalArgs a) {
+  const int per = a.mm * a.nl;
+  int idx = blockIdx.x * kThreads + threadIdx.x;
+  const int nwind = a.kx * per;
+  if (idx < nwind) {
+    const int k = idx / per, rem = idx - k * per, mode = rem / a.nl, l = rem - mode * a.nl;
+    const int m = (mode + 1) >> 1;
+    const float* z = a.vor + (size_t)k * per;
+    const float* d = a.div + (size_t)k * per;
+    float u = 0.0f, v = 0.0f;
+    if (l < a.nl - 1) {     // the highest total wavenumber is clipped
+      auto inv_lap = [&](int ll) { return ll == 0 ? 0.0f : -a.radius2 / ((float)ll * (float)(ll + 1)); };
+      // d/dlon of psi, chi at (mode, l)
+      const float psi_lon = d_dlon(z, mode, m, l, a.nl) * inv_lap(l);
+      const float chi_lon = d_dlon(d, mode, m, l, a.nl) * inv_lap(l);
+      // cos(lat) d/dlat at l: (l+2) eps(l+1) x_{l+1} - (l-1) eps(l) x_{l-1}
+      float psi_lat = 0.0f, chi_lat = 0.0f;
+      if (l + 1 < a.nl) {
+        const float c = (float)(l + 2) * eps_ml(m, l + 1) * inv_lap(l + 1);
+        psi_lat += c * z[mode * a.nl + l + 1];
+        chi_lat += c * d[mode * a.nl + l + 1];
+      }
+      if (l >= 1) {
+        const float c = -(float)(l - 1) * eps_ml(m, l) * inv_lap(l - 1);
+        psi_lat += c * z[mode * a.nl + l - 1];
+        chi_lat += c * d[mode * a.nl + l - 1];
+      }
+      u = (chi_lon - psi_lat) * a.inv_radius;
+      v = (chi_lat + psi_lon) * a.inv_radius;
+    }
+    a.out[(size_t)k * per + rem] = u;
+    a.out[(size_t)(a.kx + k) * per + rem] = v;
+    return;
+  }
+  idx -= nwind;
+  size_t row = 2 * (size_t)a.kx;
+  for (int s = 0; s < 4; ++s) {
+    const int n = a.copy_count[s] * per;
+    if (idx < n) {
+      a.out[row * per + idx] = a.copy_src[s] ? a.copy_src[s][idx] : 0.0f;
+      return;
+    }
+    idx -= n;
+    row += a.copy_count[s];
+  }
+}
+
+// Modal (U, V) = (u, v)/cos(lat) -> vorticity and divergence (curl_cos_lat,
+// div_cos_lat, clipped), then the remaining stacked rows to their destinations
+// (null source rows: zeros).
+__global__ __launch_bounds__(kThreads) void modal_vordiv_kernel(ModalArgs a) {
+  const int per = a.mm * a.nl;
+  int idx = blockIdx.x * kThreads + threadIdx.x;
+  const int nwind = a.kx * per;
+  if (idx < nwind) {
+    const int k = idx / per, rem = idx - k * per, mode = rem / a.nl, l = rem - mode * a.nl;
+    const int m = (mode + 1) >> 1;
+    const float* U = a.out + (size_t)k * per;
+    const float* V = a.out + (size_t)(a.kx + k) * per;
+    float z = 0.0f, d = 0.0f;
+    if (l < a.nl - 1) {
+      // sec(lat) d/dlat cos^2(lat) at l: l eps(l+1) x_{l+1} - (l+1) eps(l) x_{l-1}
+      float u_lat = 0.0f, v_lat = 0.0f;
+      if (l + 1 < a.nl) {
+        const float c = (float)l * eps_ml(m, l + 1);
+        u_lat += c * U[mode * a.nl + l + 1];
+        v_lat += c * V[mode * a.nl + l + 1];
+      }
+      if (l >= 1) {
+        const float c = -(float)(l + 1) * eps_ml(m, l);
+        u_lat += c * U[mode * a.nl + l - 1];
+        v_lat += c * V[mode * a.nl + l - 1];
+      }
+      z = (d_dlon(V, mode, m, l, a.nl) - u_lat) * a.inv_radius;
+      d = (d_dlon(U, mode, m, l, a.nl) + v_lat) * a.inv_radius;
+    }
+    a.vor_out[(size_t)k * per + rem] = z;
+    a.div_out[(size_t)k * per + rem] = d;
+    return;
+  }
+  idx -= nwind;
+  size_t row = 2 * (size_t)a.kx;
+  for (int s = 0; s < 4; ++s) {
+    const int n = a.copy_count[s] * per;
+    if (idx < n) {
+      a.copy_dst[s][idx] = a.copy_src[s] ? a.out[row * per + idx] : 0.0f;
+      return;
+    }
+    idx -= n;
+    row += a.copy_count[s];
+  }
+}
+
+struct GeoArgs {
+  int32_t kx, npt;
+  float w[SPEEDY_SHT_KX_MAX][SPEEDY_SHT_KX_MAX];
+  const float* phis;   // [npt] g * orography
+  const float* t;      // [kx][npt]
+  float* phi;          // [kx][npt]
+};
+
+// get_geopotential: phi_j = phi_s + sum_k W[j][k] T_k (hydrostatic trapezoid on log sigma).
+__global__ __launch_bounds__(kThreads) void geopotential_kernel(GeoArgs a) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= a.npt) return;
+  float t[SPEEDY_SHT_KX_MAX];
+#pragma unroll
+  for (int k = 0; k < SPEEDY_SHT_KX_MAX; ++k) t[k] = k < a.kx ? a.t[(size_t)k * a.npt + p] : 0.0f;
+  const float s = a.phis[p];
+  for (int j = 0; j < a.kx; ++j) {
+    float acc = s;
+#pragma unroll
+    for (int k = 0; k < SPEEDY_SHT_KX_MAX; ++k) acc += a.w[j][k] * t[k];
+    a.phi[(size_t)j * a.npt + p] = acc;
+  }
+}
+
+// ------------------------------------------------------------------ host side
+double legendre_p(int n, double x, double* dp) {
+  double p0 = 1.0, p1 = x;
+  for (int k = 2; k <= n; ++k) {
+    const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+    p0 = p1;
+    p1 = p2;
+  }
+  if (n == 0) {
+    *dp = 0.0;
+    return 1.0;
+  }
+  *dp = n * (x * p1 - p0) / (x * x - 1.0);
+  return p1;
+}
+
+void gauss_nodes(int n, double* x, double* w) {
+  for (int i = 0; i < n; ++i) {
+    // i-th root from the north (cos guess), stored south to north
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      const double p = legendre_p(n, z, &dp);
+      const double dz = p / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    legendre_p(n, z, &dp);
+    x[n - 1 - i] = z;
+    w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+int check_grid(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat) {
+  if (n_m < 1 || n_l < 1 || nlon < 1 || nlat < 2 || n_m > n_l || 2 * n_m - 1 > nlon || n_l > nlat ||
+      (int64_t)n_m * nlat * n_l >= ((int64_t)1 << 31) || (int64_t)nlon * nlat * 64 >= ((int64_t)1 << 31))
+    return fail(SPEEDY_ERR_ARG, "speedy_sht: bad grid (need 1 <= n_m <= n_l <= nlat, 2*n_m-1 <= nlon)");
+  return SPEEDY_OK;
+}
+
+void fill_basis(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, const double* x, float* fourier,
+                float* legendre) {
+  const int mm = 2 * n_m - 1;
+  if (fourier) {
+    for (int i = 0; i < nlon; ++i) {
+      const double lon = 2.0 * M_PI * i / nlon;
+      fourier[(size_t)i * mm] = (float)(1.0 / std::sqrt(2.0 * M_PI));
+      for (int m = 1; m < n_m; ++m) {
+        fourier[(size_t)i * mm + 2 * m - 1] = (float)(std::cos(m * lon) / std::sqrt(M_PI));
+        fourier[(size_t)i * mm + 2 * m] = (float)(std::sin(m * lon) / std::sqrt(M_PI));
+      }
+    }
+  }
+  if (legendre) {
+    std::vector<double> col(n_l);
+    for (int j = 0; j < nlat; ++j) {
+      const double xj = x[j], yj = std::sqrt(1.0 - xj * xj);
+      double diag = 1.0 / std::sqrt(2.0);
+      for (int m = 0; m < n_m; ++m) {
+        if (m > 0) diag = -std::sqrt(1.0 + 1.0 / (2.0 * m)) * yj * diag;
+        std::fill(col.begin(), col.end(), 0.0);
+        col[m] = diag;
+        if (m + 1 < n_l) col[m + 1] = std::sqrt(2.0 * m + 3.0) * xj * diag;
+        for (int l = m + 2; l < n_l; ++l) {
+          const double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+          const double b = std::sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+          col[l] = a * (xj * col[l - 1] - b * col[l - 2]);
+        }
+        float* dst = legendre + ((size_t)m * nlat + j) * n_l;
+        for (int l = 0; l < n_l; ++l) dst[l] = (float)col[l];
+      }
+    }
+  }
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+size_t ws_bytes(const SpeedyShtPlan* p, int32_t nfield) {
+  return align256((size_t)nfield * p->mm * p->nlat * 4) + align256((size_t)nfield * p->mm * p->n_l * 4) +
+         align256((size_t)p->nlon * p->nlat * 4);
+}
+
+struct Ws {
+  float* y;
+  float* z;
+  float* phis;
+};
+
+Ws carve(const SpeedyShtPlan* p, int32_t nfield, void* ws) {
+  char* b = static_cast<char*>(ws);
+  Ws r;
+  r.y = reinterpret_cast<float*>(b);
+  b += align256((size_t)nfield * p->mm * p->nlat * 4);
+  r.z = reinterpret_cast<float*>(b);
+  b += align256((size_t)nfield * p->mm * p->n_l * 4);
+  r.phis = reinterpret_cast<float*>(b);
+  return r;
+}
+
+StageArgs stage_args(const SpeedyShtPlan* p, int32_t nfield) {
+  StageArgs a{};
+  a.nlon = p->nlon; a.nlat = p->nlat; a.mm = p->mm; a.nl = p->n_l; a.nfield = nfield;
+  a.F = p->F; a.P = p->P; a.w = p->w; a.sec = p->sec;
+  return a;
+}
+
+int tiles(int M, int N) { return ((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN); }
+
+hipError_t run_analysis(const SpeedyShtPlan* p, StageArgs a, float* y, float* modal_out, hipStream_t s) {
+  a.out = y;
+  hipLaunchKernelGGL(sht_gemm_kernel<kFourierAnalysis>, dim3(tiles(p->mm, p->nlat), a.nfield), dim3(kThreads), 0, s,
+                     a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  a.in = y;
+  a.out = modal_out;
+  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreAnalysis>, dim3(tiles(a.nfield, p->n_l), p->mm), dim3(kThreads), 0, s,
+                     a);
+  return hipGetLastError();
+}
+
+hipError_t run_synthesis(const SpeedyShtPlan* p, StageArgs a, const float* modal_in, float* y, hipStream_t s) {
+  a.in = modal_in;
+  a.out = y;
+  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreSynthesis>, dim3(tiles(a.nfield, p->nlat), p->mm), dim3(kThreads), 0,
+                     s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  a.in = y;
+  a.out = nullptr;
+  hipLaunchKernelGGL(sht_gemm_kernel<kFourierSynthesis>, dim3(tiles(p->nlon, p->nlat), a.nfield), dim3(kThreads), 0,
+                     s, a);
+  return hipGetLastError();
+}
+
+void add_seg(SegTable& t, const float* src, float* dst, int32_t count, int32_t op, float mul,
+             const float* level = nullptr, int32_t period = 1) {
+  Seg& s = t.s[t.n];
+  s.src = src; s.dst = dst; s.level = level; s.period = period; s.op = op; s.mul = mul; s.count = count;
+  s.first = t.n == 0 ? 0 : t.s[t.n - 1].first + t.s[t.n - 1].count;
+  ++t.n;
+}
+
+int check_call(const SpeedyShtPlan* p, int32_t nfield, void* ws, size_t wsb) {
+  if (!p) return fail(SPEEDY_ERR_ARG, "speedy_sht: null plan");
+  if (nfield < 1 || nfield > 65535) return fail(SPEEDY_ERR_ARG, "speedy_sht: nfield must be in 1..65535");
+  if (!ws || wsb < ws_bytes(p, nfield))
+    return fail(SPEEDY_ERR_WORKSPACE, "speedy_sht: workspace missing or smaller than speedy_sht_workspace_bytes");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
+    return fail(SPEEDY_ERR_ARG, "speedy_sht: the plan belongs to another device than the current one");
+  return SPEEDY_OK;
+}
+
+int check_spec(const SpeedyDynamicsSpec* sp) {
+  if (!sp || sp->kx < 1 || sp->kx > SPEEDY_SHT_KX_MAX || !sp->reference_temperature)
+    return fail(SPEEDY_ERR_ARG, "speedy_sht: bad dynamics spec (kx 1..16, reference temperature)");
+  return SPEEDY_OK;
+}
+
+int hip_result(hipError_t e) {
+  if (e != hipSuccess) return fail(SPEEDY_ERR_HIP, std::string("speedy_sht: launch failed: ") + hipGetErrorString(e));
+  return SPEEDY_OK;
+}
+
+int blocks(size_t n) { return (int)((n + kThreads - 1) / kThreads); }
+
+}  // namespace sht
+
+using namespace sht;
+
+extern "C" {
+
+int speedy_sht_host_basis(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, double* sin_lat, double* weights,
+                          float* fourier, float* legendre) {
+  if (int rc = check_grid(n_m, n_l, nlon, nlat)) return rc;
+  if (!sin_lat || !weights) return fail(SPEEDY_ERR_ARG, "speedy_sht_host_basis: null nodes or weights");
+  gauss_nodes(nlat, sin_lat, weights);
+  fill_basis(n_m, n_l, nlon, nlat, sin_lat, fourier, legendre);
+  return SPEEDY_OK;
+}
+
+int speedy_sht_create(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, double radius, SpeedyShtPlan** out) {
+  if (!out) return fail(SPEEDY_ERR_ARG, "speedy_sht_create: null output");
+  *out = nullptr;
+  if (int rc = check_grid(n_m, n_l, nlon, nlat)) return rc;
+  if (!(radius > 0.0)) return fail(SPEEDY_ERR_ARG, "speedy_sht_create: radius must be positive");
+  const int mm = 2 * n_m - 1;
+  std::vector<double> x(nlat), w(nlat);
+  gauss_nodes(nlat, x.data(), w.data());
+  std::vector<float> F((size_t)nlon * mm), P((size_t)n_m * nlat * n_l), wf(nlat), sec(nlat);
+  fill_basis(n_m, n_l, nlon, nlat, x.data(), F.data(), P.data());
+  for (int j = 0; j < nlat; ++j) {
+    wf[j] = (float)(w[j] * 2.0 * M_PI / nlon);
+    sec[j] = (float)(1.0 / std::sqrt(1.0 - x[j] * x[j]));
+  }
+  SpeedyShtPlan* p = new SpeedyShtPlan{n_m, n_l, nlon, nlat, mm, radius, 0, nullptr, nullptr, nullptr, nullptr};
+  hipError_t e = hipGetDevice(&p->device);
+  auto up = [&](float** dst, const std::vector<float>& v) {
+    if (e != hipSuccess) return;
+    e = hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice);
+  };
+  up(&p->F, F);
+  up(&p->P, P);
+  up(&p->w, wf);
+  up(&p->sec, sec);
+  if (e != hipSuccess) {
+    speedy_sht_destroy(p);
+    return fail(SPEEDY_ERR_HIP, std::string("speedy_sht_create: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return SPEEDY_OK;
+}
+
+int speedy_sht_destroy(SpeedyShtPlan* p) {
+  if (!p) return SPEEDY_OK;
+  for (float* q : {p->F, p->P, p->w, p->sec})
+    if (q) (void)hipFree(q);
+  delete p;
+  return SPEEDY_OK;
+}
+
+size_t speedy_sht_workspace_bytes(const SpeedyShtPlan* p, int32_t nfield) {
+  return p && nfield > 0 ? ws_bytes(p, nfield) : 0;
+}
+
+int speedy_sht_to_modal(const SpeedyShtPlan* p, const float* nodal, float* modal, int32_t nfield, void* ws,
+                        size_t wsb, void* stream) {
+  if (int rc = check_call(p, nfield, ws, wsb)) return rc;
+  if (!nodal || !modal) return fail(SPEEDY_ERR_ARG, "speedy_sht_to_modal: null field");
+  StageArgs a = stage_args(p, nfield);
+  add_seg(a.seg, nodal, nullptr, nfield, 0, 1.0f);
+  const Ws w = carve(p, nfield, ws);
+  return hip_result(run_analysis(p, a, w.y, modal, static_cast<hipStream_t>(stream)));
+}
+
+int speedy_sht_to_nodal(const SpeedyShtPlan* p, const float* modal, float* nodal, int32_t nfield, void* ws,
+                        size_t wsb, void* stream) {
+  if (int rc = check_call(p, nfield, ws, wsb)) return rc;
+  if (!nodal || !modal) return fail(SPEEDY_ERR_ARG, "speedy_sht_to_nodal: null field");
+  StageArgs a = stage_args(p, nfield);
+  add_seg(a.seg, nullptr, nodal, nfield, 0, 1.0f);
+  const Ws w = carve(p, nfield, ws);
+  return hip_result(run_synthesis(p, a, modal, w.y, static_cast<hipStream_t>(stream)));
+}
+
+int speedy_dynamics_to_physics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec* spec, const SpeedySpectralState* in,
+                               const SpeedyNodalState* out, void* ws, size_t wsb, void* stream) {
+  if (int rc = check_spec(spec)) return rc;
+  if (!in || !out || !in->vorticity || !in->divergence || !in->temperature_variation || !in->specific_humidity ||
+      !in->log_surface_pressure || !spec->orography || !spec->sigma_boundaries || !out->u_wind || !out->v_wind ||
+      !out->temperature || !out->specific_humidity || !out->geopotential || !out->normalized_surface_pressure)
+    return fail(SPEEDY_ERR_ARG, "speedy_dynamics_to_physics: null field");
+  const int kx = spec->kx, nsp = in->nsp;
+  if (nsp < 1 || nsp > kx || out->nsp != nsp)
+    return fail(SPEEDY_ERR_ARG, "speedy_dynamics_to_physics: nsp must be in 1..kx and equal on both sides");
+  const int nfield = 4 * kx + nsp + 1;
+  if (int rc = check_call(p, nfield, ws, wsb)) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const Ws w = carve(p, nfield, ws);
+  // 1. modal (u cos, v cos) and the other modal inputs, stacked in z
+  ModalArgs m{};
+  m.mm = p->mm; m.nl = p->n_l; m.kx = kx;
+  m.inv_radius = (float)(1.0 / p->radius);
+  m.radius2 = (float)(p->radius * p->radius);
+  m.vor = in->vorticity; m.div = in->divergence;
+  m.copy_src[0] = in->temperature_variation; m.copy_count[0] = kx;
+  m.copy_src[1] = in->specific_humidity; m.copy_count[1] = kx;
+  m.copy_src[2] = in->log_surface_pressure; m.copy_count[2] = nsp;
+  m.copy_src[3] = spec->orography; m.copy_count[3] = 1;
+  m.out = w.z;
+  const size_t per = (size_t)p->mm * p->n_l;
+  hipLaunchKernelGGL(modal_winds_kernel, dim3(blocks(per * nfield - per * kx)), dim3(kThreads), 0, s, m);
+  if (int rc = hip_result(hipGetLastError())) return rc;
+  // 2. synthesis with the nodal post-operations
+  StageArgs a = stage_args(p, nfield);
+  add_seg(a.seg, nullptr, out->u_wind, kx, kOpSecLat, 1.0f);
+  add_seg(a.seg, nullptr, out->v_wind, kx, kOpSecLat, 1.0f);
+  add_seg(a.seg, nullptr, out->temperature, kx, kOpAddLevel, 1.0f, spec->reference_temperature, kx);
+  add_seg(a.seg, nullptr, out->specific_humidity, kx, 0, 1000.0f);
+  add_seg(a.seg, nullptr, out->normalized_surface_pressure, nsp, kOpExp, 1.0f);
+  add_seg(a.seg, nullptr, w.phis, 1, 0, (float)spec->gravity);
+  if (int rc = hip_result(run_synthesis(p, a, w.z, w.y, s))) return rc;
+  // 3. geopotential from the nodal temperature
+  GeoArgs g{};
+  g.kx = kx;
+  g.npt = p->nlon * p->nlat;
+  std::vector<double> ls(kx);
+  for (int k = 0; k < kx; ++k)
+    ls[k] = std::log(0.5 * (spec->sigma_boundaries[k] + spec->sigma_boundaries[k + 1]));
+  std::vector<double> alpha(kx);
+  for (int k = 0; k + 1 < kx; ++k) alpha[k] = 0.5 * (ls[k + 1] - ls[k]);
+  alpha[kx - 1] = -ls[kx - 1];
+  for (int j = 0; j < kx; ++j)
+    for (int k = 0; k < kx; ++k)
+      g.w[j][k] = (float)(spec->gas_constant * (k < j ? 0.0 : k == j ? alpha[j] : alpha[k] + alpha[k - 1]));
+  g.phis = w.phis;
+  g.t = out->temperature;
+  g.phi = out->geopotential;
+  hipLaunchKernelGGL(geopotential_kernel, dim3(blocks(g.npt)), dim3(kThreads), 0, s, g);
+  return hip_result(hipGetLastError());
+}
+
+namespace {
+int nodal_to_spectral(const SpeedyShtPlan* p, const SpeedyDynamicsSpec* spec, const SpeedyNodalState* in,
+                      const SpeedySpectralState* out, void* ws, size_t wsb, void* stream, bool tendency) {
+  const char* who = tendency ? "speedy_physics_tendency_to_dynamics" : "speedy_physics_to_dynamics";
+  if (int rc = check_spec(spec)) return rc;
+  if (!in || !out || !in->u_wind || !in->v_wind || !in->temperature || !in->specific_humidity ||
+      (!tendency && !in->normalized_surface_pressure) || !out->vorticity || !out->divergence ||
+      !out->temperature_variation || !out->specific_humidity || !out->log_surface_pressure)
+    return fail(SPEEDY_ERR_ARG, std::string(who) + ": null field");
+  const int kx = spec->kx, nsp = out->nsp;
+  if (nsp < 1 || nsp > kx || (!tendency && in->nsp != nsp))
+    return fail(SPEEDY_ERR_ARG, std::string(who) + ": nsp must be in 1..kx and equal on both sides");
+  const int nsp_in = tendency ? 0 : nsp;
+  const int nfield = 4 * kx + nsp_in;
+  if (int rc = check_call(p, nfield, ws, wsb)) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const Ws w = carve(p, nfield, ws);
+  StageArgs a = stage_args(p, nfield);
+  add_seg(a.seg, in->u_wind, nullptr, kx, kOpSecLat, 1.0f);
+  add_seg(a.seg, in->v_wind, nullptr, kx, kOpSecLat, 1.0f);
+  if (tendency)
+    add_seg(a.seg, in->temperature, nullptr, kx, 0, 1.0f);
+  else
+    add_seg(a.seg, in->temperature, nullptr, kx, kOpSubLevel, 1.0f, spec->reference_temperature, kx);
+  add_seg(a.seg, in->specific_humidity, nullptr, kx, 0, 1e-3f);
+  if (!tendency) add_seg(a.seg, in->normalized_surface_pressure, nullptr, nsp, kOpLog, 1.0f);
+  if (int rc = hip_result(run_analysis(p, a, w.y, w.z, s))) return rc;
+  ModalArgs m{};
+  m.mm = p->mm; m.nl = p->n_l; m.kx = kx;
+  m.inv_radius = (float)(1.0 / p->radius);
+  m.radius2 = (float)(p->radius * p->radius);
+  m.out = w.z;
+  m.vor_out = out->vorticity; m.div_out = out->divergence;
+  m.copy_src[0] = w.z; m.copy_count[0] = kx; m.copy_dst[0] = out->temperature_variation;
+  m.copy_src[1] = w.z; m.copy_count[1] = kx; m.copy_dst[1] = out->specific_humidity;
+  m.copy_src[2] = tendency ? nullptr : w.z; m.copy_count[2] = nsp; m.copy_dst[2] = out->log_surface_pressure;
+  const size_t per = (size_t)p->mm * p->n_l;
+  hipLaunchKernelGGL(modal_vordiv_kernel, dim3(blocks(per * (3 * kx + nsp))), dim3(kThreads), 0, s, m);
+  return hip_result(hipGetLastError());
+}
+}  // namespace
+
+int speedy_physics_to_dynamics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec* spec, const SpeedyNodalState* in,
+                               const SpeedySpectralState* out, void* ws, size_t wsb, void* stream) {
+  return nodal_to_spectral(p, spec, in, out, ws, wsb, stream, false);
+}
+
+int speedy_physics_tendency_to_dynamics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec* spec,
+                                        const SpeedyNodalState* in, const SpeedySpectralState* out, void* ws,
+                                        size_t wsb, void* stream) {
+  return nodal_to_spectral(p, spec, in, out, ws, wsb, stream, true);
+}
+
+const char* speedy_sht_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -72,3 +72,21 @@ def verify_tendencies(state: PhysicsState, tendencies: PhysicsTendency, time_ste
         import numpy as np
         new = np.where(ok, dq, -q / time_step)
     return tendencies.copy(specific_humidity=new)
+
+
+def dynamics_state_to_physics_state(state, dynamics) -> PhysicsState:
+    """physics_interface.py:196-234, on the GPU (jcm_amd.transforms)."""
+    from .transforms import dynamics_state_to_physics_state as f
+    return f(state, dynamics)
+
+
+def physics_state_to_dynamics_state(physics_state: PhysicsState, dynamics):
+    """physics_interface.py:237-273, on the GPU (jcm_amd.transforms)."""
+    from .transforms import physics_state_to_dynamics_state as f
+    return f(physics_state, dynamics)
+
+
+def physics_tendency_to_dynamics_tendency(physics_tendency: PhysicsTendency, dynamics):
+    """physics_interface.py:275-303, on the GPU (jcm_amd.transforms)."""
+    from .transforms import physics_tendency_to_dynamics_tendency as f
+    return f(physics_tendency, dynamics)
