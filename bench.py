@@ -348,6 +348,66 @@ def large_leg(phys, dev, reps=5):
     return out
 
 
+MFMA_FP32_PEAK_TFS = 157.3
+
+
+def transform_work(g, kx):
+    """(flops, bytes) of one transform step: dynamics -> physics state plus physics
+    tendency -> dynamics at the grid g.  Flops: 2 per multiply-add of the Legendre
+    and Fourier contractions (5*kx+1 fields synthesised: winds, temperature,
+    humidity, log surface pressure, geopotential; 4*kx analysed).  Bytes: modal
+    inputs and outputs, nodal inputs and outputs, and each basis read once."""
+    mm, nl, nlon, nlat = 2 * g.n_m - 1, g.n_l, g.nlon, g.nlat
+    contraction = lambda nf: 2.0 * nf * mm * nlat * (nl + nlon)
+    flops = contraction(5 * kx + 1) + contraction(4 * kx)
+    modal, nodal = mm * nl * 4, nlon * nlat * 4
+    bases = g.n_m * nlat * nl * 4 + nlon * mm * 4
+    nbytes = ((4 * kx + 2) * modal + (5 * kx + 1) * nodal      # dynamics -> physics
+              + 4 * kx * nodal + (4 * kx + 1) * modal           # tendency -> dynamics
+              + 2 * bases)
+    return flops, nbytes
+
+
+def transforms_leg(dev, trunc=85, kx=8, reps=None, calls=20, profile_reps=0):
+    """T85 x 8 levels: dynamics_state_to_physics_state + physics_tendency_to_dynamics_tendency
+    per step (the conversions around one physics call), replayed from a HIP graph."""
+    import numpy as np
+    import torch
+    from jcm_amd import transforms as T
+    from jcm_amd.geometry import SIGMA_LAYER_BOUNDARIES
+    from jcm_amd.physics_interface import PhysicsTendency
+    g = T.SpectralGrid.truncation(trunc)
+    rng = np.random.default_rng(11)
+    mode_m = np.array([0] + [m for mm in range(1, g.n_m) for m in (mm, mm)])
+    mask = (mode_m[:, None] <= np.arange(g.n_l)[None, :]).astype(np.float32)
+    modal = lambda lead, scale: torch.as_tensor(
+        (rng.standard_normal(tuple(lead) + g.modal_shape) * scale * mask).astype(np.float32), device=dev)
+    dyn = T.PrimitiveEquations(reference_temperature=torch.linspace(220.0, 288.0, kx, device=dev),
+                               orography=modal((), 300.0),
+                               coords=T.CoordinateSystem(g, T.SigmaCoordinates(SIGMA_LAYER_BOUNDARIES[kx])))
+    st = T.DynamicsState(modal((kx,), 1e-5), modal((kx,), 1e-6), modal((kx,), 5.0), modal((1,), 0.02),
+                         {"specific_humidity": modal((kx,), 1e-3)})
+    shape = (kx,) + g.nodal_shape
+    tend = PhysicsTendency(*(torch.randn(shape, device=dev) * s for s in (1e-4, 1e-4, 1e-5, 1e-6)))
+    fn = lambda: (T.dynamics_state_to_physics_state(st, dyn), T.physics_tendency_to_dynamics_tendency(tend, dyn))
+    step = GraphStep(fn, calls, dev)
+    if profile_reps:
+        for _ in range(profile_reps):
+            step.graph.replay()
+        torch.cuda.synchronize()
+        return None
+    ms = step.time(reps)
+    flops, nbytes = transform_work(g, kx)
+    tfs = flops / (ms * 1e-3) / 1e12
+    return {"workload": f"T{trunc} {g.nlon}x{g.nlat} x {kx} levels: dynamics_state_to_physics_state + "
+                        "physics_tendency_to_dynamics_tendency per step, graph replay",
+            "ms_per_step": ms, "steps_per_s": 1e3 / ms,
+            "roofline": {"bound": "mfma", "achieved": tfs, "peak": MFMA_FP32_PEAK_TFS, "unit": "TFLOP/s",
+                         "frac": tfs / MFMA_FP32_PEAK_TFS, "flops_per_step": flops, "dtype": "f32",
+                         "hbm_bytes_per_step": nbytes,
+                         "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -390,6 +450,7 @@ def main():
             g_vjp.graph.replay()
         torch.cuda.synchronize()
         large_leg(vphys, dev, reps=3)
+        transforms_leg(dev, profile_reps=max(1, steps // 100))
         # A copy of known size (torch.neg: reads and writes copy_mib MiB each) whose
         # FETCH_SIZE / WRITE_SIZE counters calibrate the counter units per byte.
         n = (args.copy_mib << 20) // 4
@@ -399,7 +460,8 @@ def main():
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
         print(f"profile run: {steps} forward and {steps} forward+VJP calls at T85 replayed from graphs of {G}; "
-              f"4 batched T425 forward+VJP calls; 3 x neg of {args.copy_mib} MiB", flush=True)
+              f"4 batched T425 forward+VJP calls; {max(1, steps // 100) * 20} T85 transform steps; "
+              f"3 x neg of {args.copy_mib} MiB", flush=True)
         return
 
     # ---- timed region (headline): K forward steps replayed from graphs
@@ -482,6 +544,7 @@ def main():
                                         "fwd_columns_per_s": ncol / (pc_pf * 1e-3),
                                         "fwd_vjp_columns_per_s": ncol / (pc_pv * 1e-3)}}
     out["large"] = large_leg(vphys, dev)
+    out["transforms"] = transforms_leg(dev, reps=max(1, steps // 20))
     if not args.no_serving:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         out["serving"] = serving_runs(vphys, dev, args.serve_requests, rates)

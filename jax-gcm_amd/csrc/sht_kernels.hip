@@ -11,7 +11,7 @@
 //   to_modal:  nodal [B][nlon][nlat] --Fourier--> Y [B][Mm][nlat] --Legendre--> modal [B][Mm][L]
 // Each arrow is one launch of sht_gemm_kernel<STAGE>: a 64x64 output tile per
 // workgroup of four waves, each wave 32x32 as 2x2 v_mfma_f32_16x16x4f32, K in
-// steps of 16 staged through LDS.  The Fourier stages batch over fields (the
+// steps of 32 staged through two LDS stages.  The Fourier stages batch over fields (the
 // basis F is shared), the Legendre stages batch over modes (the basis P[m] is
 // shared by every field).  Per-field pre/post operations (1/cos(lat), unit
 // scales, reference temperature, log/exp of surface pressure) ride in the
@@ -47,7 +47,7 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-constexpr int kThreads = 256, kBM = 64, kBN = 64, kBK = 16, kPad = 4;
+constexpr int kThreads = 256, kBM = 64, kBN = 64, kBK = 32, kPad = 4;
 constexpr int kMaxSeg = 8;
 
 enum Stage { kFourierAnalysis = 0, kLegendreAnalysis = 1, kLegendreSynthesis = 2, kFourierSynthesis = 3 };
@@ -127,14 +127,17 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
   }
   constexpr bool kAKContig = STAGE != kFourierAnalysis;
   constexpr bool kBNContig = STAGE != kLegendreSynthesis;
+  constexpr int kLoads = kBM * kBK / kThreads;
 
   const int tiles_n = (N + kBN - 1) / kBN;
   const int tiles_m = (M + kBM - 1) / kBM;
   if ((int)blockIdx.x >= tiles_m * tiles_n) return;    // uniform per workgroup
   const int m0 = (blockIdx.x / tiles_n) * kBM, n0 = (blockIdx.x % tiles_n) * kBN;
 
-  __shared__ float As[kBK][kBM + kPad];
-  __shared__ float Bs[kBK][kBN + kPad];
+  // two LDS stages: the global loads of K-step s+1 are in flight (in registers)
+  // while the MFMAs of step s read the other stage; one barrier per step
+  __shared__ float As[2][kBK][kBM + kPad];
+  __shared__ float Bs[2][kBK][kBN + kPad];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   floatx4 acc[2][2];
@@ -143,47 +146,81 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  for (int k0 = 0; k0 < K; k0 += kBK) {
+  // Fourier analysis: x -> (op(x) - level) * mul * w(lat) [* sec(lat)], applied when
+  // the prefetched values go to LDS (after the MFMAs of the current step), so the
+  // loads stay in flight; the column of a thread's B elements is fixed (kBN | kThreads).
+  float colscale = 1.0f;
+  if (STAGE == kFourierAnalysis) {
+    const int gn = n0 + t % kBN;
+    colscale = gn < N ? sg.mul * a.w[gn] * ((sg.op & kOpSecLat) ? a.sec[gn] : 1.0f) : 0.0f;
+  }
+  float ra[kLoads], rb[kLoads];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < kLoads; ++r) {
       const int e = t + kThreads * r;
-      const int mm = kAKContig ? (e >> 4) : (e & 63);
-      const int kk = kAKContig ? (e & 15) : (e >> 6);
-      const int gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < M && gk < K) ? A[gm * a_m + gk * a_k] : 0.0f;
+      const int mm = kAKContig ? (e / kBK) : (e % kBM);
+      const int kk = kAKContig ? (e % kBK) : (e / kBM);
+      // clamped indices (always in bounds), no condition: the loads issue back to
+      // back; elements outside the matrix are zeroed when they go to LDS
+      ra[r] = A[min(m0 + mm, M - 1) * a_m + min(k0 + kk, K - 1) * a_k];
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < kLoads; ++r) {
       const int e = t + kThreads * r;
-      const int nn = kBNContig ? (e & 63) : (e >> 4);
-      const int kk = kBNContig ? (e >> 6) : (e & 15);
-      const int gk = k0 + kk, gn = n0 + nn;
-      float v = 0.0f;
-      if (gk < K && gn < N) {
-        v = B[gk * b_k + gn * b_n];
-        if (STAGE == kFourierAnalysis) {
-          if (sg.op & kOpLog) v = logf(v);
-          if (sg.op & kOpSubLevel) v -= lvl;
-          v *= sg.mul * a.w[gn];
-          if (sg.op & kOpSecLat) v *= a.sec[gn];
-        }
+      const int nn = kBNContig ? (e % kBN) : (e / kBK);
+      const int kk = kBNContig ? (e / kBN) : (e % kBK);
+      rb[r] = B[min(k0 + kk, K - 1) * b_k + min(n0 + nn, N - 1) * b_n];
+    }
+  };
+  auto store = [&](int st, int k0) {
+#pragma unroll
+    for (int r = 0; r < kLoads; ++r) {
+      const int e = t + kThreads * r;
+      const int am = kAKContig ? (e / kBK) : (e % kBM);
+      const int ak = kAKContig ? (e % kBK) : (e / kBM);
+      As[st][ak][am] = (m0 + am < M && k0 + ak < K) ? ra[r] : 0.0f;
+      const int nn = kBNContig ? (e % kBN) : (e / kBK);
+      const int kk = kBNContig ? (e / kBN) : (e % kBK);
+      float v = rb[r];
+      if (STAGE == kFourierAnalysis) {
+        if (sg.op & kOpLog) v = logf(v);
+        if (sg.op & kOpSubLevel) v -= lvl;
+        v *= colscale;
       }
-      Bs[kk][nn] = v;
+      Bs[st][kk][nn] = (k0 + kk < K && n0 + nn < N) ? v : 0.0f;
     }
-    __syncthreads();
+  };
+  load(0);
+  store(0, 0);
+  __syncthreads();
+  int st = 0;
+  for (int k0 = 0; k0 < K; k0 += kBK) {
+    const bool more = k0 + kBK < K;
+    if (more) load(k0 + kBK);
+    // all operands of this step out of LDS first, then the MFMAs
+    float av[kBK / 4][2], bv[kBK / 4][2];
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 4) {
-      float av[2], bv[2];
+    for (int q = 0; q < kBK / 4; ++q) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) av[i] = As[kk + (lane >> 4)][wm + 16 * i + (lane & 15)];
+      for (int i = 0; i < 2; ++i) av[q][i] = As[st][4 * q + (lane >> 4)][wm + 16 * i + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bv[j] = Bs[kk + (lane >> 4)][wn + 16 * j + (lane & 15)];
+      for (int j = 0; j < 2; ++j) bv[q][j] = Bs[st][4 * q + (lane >> 4)][wn + 16 * j + (lane & 15)];
+    }
+#pragma unroll
+    for (int q = 0; q < kBK / 4; ++q)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
-    }
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][i], bv[q][j], acc[i][j], 0, 0, 0);
+    // keep the LDS reads ahead of the MFMAs (the default schedule waits on each
+    // substep's read right before its four MFMAs)
+    __builtin_amdgcn_sched_group_barrier(0x100, kBK / 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, kBK, 0);
+    if (more) store(st ^ 1, k0 + kBK);
     __syncthreads();
+    st ^= 1;
   }
   // lane holds C[16 i + 4 (lane >> 4) + r][16 j + (lane & 15)] of its wave's 32 x 32
 #pragma unroll
@@ -227,7 +264,18 @@ struct ModalArgs {
   float* vor_out;       // post: destinations of vorticity / divergence
   float* div_out;
   float* copy_dst[4];
+  // geopotential rows behind the copies (winds kernel; kx rows when geo_w is set):
+  // phi_j = g orography + sum_k W[j][k] (T_var_k + reference_k as a constant)
+  const float* tvar;
+  const float* orography;
+  const float* reference;
+  float gravity;
+  int32_t geo;
+  float geo_w[SPEEDY_SHT_KX_MAX][SPEEDY_SHT_KX_MAX];
 };
+
+// Modal coefficient (mode 0, l 0) of the constant 1: 1 / (F_0 P_0^0) = 2 sqrt(pi).
+constexpr float kConstCoef = 3.5449077018110318f;
 
 __device__ __forceinline__ float d_dlon(const float* x, int mode, int m, int l, int nl) {
   if (mode == 0) return 0.0f;
@@ -282,6 +330,17 @@ __global__ __launch_bounds__(kThreads) void modal_winds_kernel(ModalArgs a) {
     idx -= n;
     row += a.copy_count[s];
   }
+  if (!a.geo || idx >= a.kx * per) return;
+  // get_geopotential in modal space: the reference temperature enters the
+  // (mode 0, l 0) coefficient only
+  const int j = idx / per, rem = idx - j * per;
+  float acc = a.gravity * a.orography[rem];
+  for (int k = 0; k < a.kx; ++k) {
+    float tk = a.tvar[(size_t)k * per + rem];
+    if (rem == 0) tk += a.reference[k] * kConstCoef;
+    acc += a.geo_w[j][k] * tk;
+  }
+  a.out[(row + j) * per + rem] = acc;
 }
 
 // Modal (U, V) = (u, v)/cos(lat) -> vorticity and divergence (curl_cos_lat,
@@ -327,30 +386,6 @@ __global__ __launch_bounds__(kThreads) void modal_vordiv_kernel(ModalArgs a) {
     }
     idx -= n;
     row += a.copy_count[s];
-  }
-}
-
-struct GeoArgs {
-  int32_t kx, npt;
-  float w[SPEEDY_SHT_KX_MAX][SPEEDY_SHT_KX_MAX];
-  const float* phis;   // [npt] g * orography
-  const float* t;      // [kx][npt]
-  float* phi;          // [kx][npt]
-};
-
-// get_geopotential: phi_j = phi_s + sum_k W[j][k] T_k (hydrostatic trapezoid on log sigma).
-__global__ __launch_bounds__(kThreads) void geopotential_kernel(GeoArgs a) {
-  const int p = blockIdx.x * kThreads + threadIdx.x;
-  if (p >= a.npt) return;
-  float t[SPEEDY_SHT_KX_MAX];
-#pragma unroll
-  for (int k = 0; k < SPEEDY_SHT_KX_MAX; ++k) t[k] = k < a.kx ? a.t[(size_t)k * a.npt + p] : 0.0f;
-  const float s = a.phis[p];
-  for (int j = 0; j < a.kx; ++j) {
-    float acc = s;
-#pragma unroll
-    for (int k = 0; k < SPEEDY_SHT_KX_MAX; ++k) acc += a.w[j][k] * t[k];
-    a.phi[(size_t)j * a.npt + p] = acc;
   }
 }
 
@@ -431,24 +466,19 @@ void fill_basis(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, const doub
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 size_t ws_bytes(const SpeedyShtPlan* p, int32_t nfield) {
-  return align256((size_t)nfield * p->mm * p->nlat * 4) + align256((size_t)nfield * p->mm * p->n_l * 4) +
-         align256((size_t)p->nlon * p->nlat * 4);
+  return align256((size_t)nfield * p->mm * p->nlat * 4) + align256((size_t)nfield * p->mm * p->n_l * 4);
 }
 
 struct Ws {
-  float* y;
-  float* z;
-  float* phis;
+  float* y;   // [nfield][mm][nlat]: between the Fourier and Legendre stages
+  float* z;   // [nfield][mm][nl]: stacked modal fields
 };
 
 Ws carve(const SpeedyShtPlan* p, int32_t nfield, void* ws) {
   char* b = static_cast<char*>(ws);
   Ws r;
   r.y = reinterpret_cast<float*>(b);
-  b += align256((size_t)nfield * p->mm * p->nlat * 4);
-  r.z = reinterpret_cast<float*>(b);
-  b += align256((size_t)nfield * p->mm * p->n_l * 4);
-  r.phis = reinterpret_cast<float*>(b);
+  r.z = reinterpret_cast<float*>(b + align256((size_t)nfield * p->mm * p->nlat * 4));
   return r;
 }
 
@@ -610,11 +640,12 @@ int speedy_dynamics_to_physics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec*
   const int kx = spec->kx, nsp = in->nsp;
   if (nsp < 1 || nsp > kx || out->nsp != nsp)
     return fail(SPEEDY_ERR_ARG, "speedy_dynamics_to_physics: nsp must be in 1..kx and equal on both sides");
-  const int nfield = 4 * kx + nsp + 1;
+  const int nfield = 5 * kx + nsp;
   if (int rc = check_call(p, nfield, ws, wsb)) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const Ws w = carve(p, nfield, ws);
-  // 1. modal (u cos, v cos) and the other modal inputs, stacked in z
+  // 1. modal (u cos, v cos), the other modal inputs and the modal geopotential,
+  //    stacked in z: rows [u kx][v kx][T_var kx][q kx][log ps nsp][phi kx]
   ModalArgs m{};
   m.mm = p->mm; m.nl = p->n_l; m.kx = kx;
   m.inv_radius = (float)(1.0 / p->radius);
@@ -623,10 +654,23 @@ int speedy_dynamics_to_physics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec*
   m.copy_src[0] = in->temperature_variation; m.copy_count[0] = kx;
   m.copy_src[1] = in->specific_humidity; m.copy_count[1] = kx;
   m.copy_src[2] = in->log_surface_pressure; m.copy_count[2] = nsp;
-  m.copy_src[3] = spec->orography; m.copy_count[3] = 1;
   m.out = w.z;
+  m.tvar = in->temperature_variation;
+  m.orography = spec->orography;
+  m.reference = spec->reference_temperature;
+  m.gravity = (float)spec->gravity;
+  m.geo = 1;
+  // hydrostatic trapezoid on log(sigma): alpha_k = (ln s_k+1 - ln s_k)/2,
+  // alpha_K-1 = -ln s_K-1; W[j][j] = alpha_j, W[j][k>j] = alpha_k + alpha_k-1
+  std::vector<double> ls(kx), alpha(kx);
+  for (int k = 0; k < kx; ++k) ls[k] = std::log(0.5 * (spec->sigma_boundaries[k] + spec->sigma_boundaries[k + 1]));
+  for (int k = 0; k + 1 < kx; ++k) alpha[k] = 0.5 * (ls[k + 1] - ls[k]);
+  alpha[kx - 1] = -ls[kx - 1];
+  for (int j = 0; j < kx; ++j)
+    for (int k = 0; k < kx; ++k)
+      m.geo_w[j][k] = (float)(spec->gas_constant * (k < j ? 0.0 : k == j ? alpha[j] : alpha[k] + alpha[k - 1]));
   const size_t per = (size_t)p->mm * p->n_l;
-  hipLaunchKernelGGL(modal_winds_kernel, dim3(blocks(per * nfield - per * kx)), dim3(kThreads), 0, s, m);
+  hipLaunchKernelGGL(modal_winds_kernel, dim3(blocks(per * (nfield - kx))), dim3(kThreads), 0, s, m);
   if (int rc = hip_result(hipGetLastError())) return rc;
   // 2. synthesis with the nodal post-operations
   StageArgs a = stage_args(p, nfield);
@@ -635,26 +679,8 @@ int speedy_dynamics_to_physics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec*
   add_seg(a.seg, nullptr, out->temperature, kx, kOpAddLevel, 1.0f, spec->reference_temperature, kx);
   add_seg(a.seg, nullptr, out->specific_humidity, kx, 0, 1000.0f);
   add_seg(a.seg, nullptr, out->normalized_surface_pressure, nsp, kOpExp, 1.0f);
-  add_seg(a.seg, nullptr, w.phis, 1, 0, (float)spec->gravity);
-  if (int rc = hip_result(run_synthesis(p, a, w.z, w.y, s))) return rc;
-  // 3. geopotential from the nodal temperature
-  GeoArgs g{};
-  g.kx = kx;
-  g.npt = p->nlon * p->nlat;
-  std::vector<double> ls(kx);
-  for (int k = 0; k < kx; ++k)
-    ls[k] = std::log(0.5 * (spec->sigma_boundaries[k] + spec->sigma_boundaries[k + 1]));
-  std::vector<double> alpha(kx);
-  for (int k = 0; k + 1 < kx; ++k) alpha[k] = 0.5 * (ls[k + 1] - ls[k]);
-  alpha[kx - 1] = -ls[kx - 1];
-  for (int j = 0; j < kx; ++j)
-    for (int k = 0; k < kx; ++k)
-      g.w[j][k] = (float)(spec->gas_constant * (k < j ? 0.0 : k == j ? alpha[j] : alpha[k] + alpha[k - 1]));
-  g.phis = w.phis;
-  g.t = out->temperature;
-  g.phi = out->geopotential;
-  hipLaunchKernelGGL(geopotential_kernel, dim3(blocks(g.npt)), dim3(kThreads), 0, s, g);
-  return hip_result(hipGetLastError());
+  add_seg(a.seg, nullptr, out->geopotential, kx, 0, 1.0f);
+  return hip_result(run_synthesis(p, a, w.z, w.y, s));
 }
 
 namespace {

@@ -332,7 +332,7 @@ def dynamics_state_to_physics_state(state: DynamicsState, dynamics: PrimitiveEqu
     shape = (kx,) + g.nodal_shape
     u, v, t, qn, phi = (torch.empty(shape, device=dev) for _ in range(5))
     sp = torch.empty((nsp,) + g.nodal_shape, device=dev)
-    nfield = 4 * kx + nsp + 1
+    nfield = 5 * kx + nsp
     plan, ws, s = g.workspace(dev, nfield)
     ins = _Spectral(vor.data_ptr(), div.data_ptr(), tv.data_ptr(), lsp.data_ptr(), q.data_ptr(), nsp)
     outs = _Nodal(u.data_ptr(), v.data_ptr(), t.data_ptr(), qn.data_ptr(), phi.data_ptr(), sp.data_ptr(), nsp)
