@@ -10,6 +10,7 @@ writes).  There is no CPU path: without the library or a GPU the call raises.
 """
 from __future__ import annotations
 
+import copy
 import ctypes
 from types import SimpleNamespace
 from typing import Any, Tuple
@@ -60,12 +61,13 @@ class PhysicsData:
     ``data.shortwave_rad.rsds``, ``data.mod_radcon.tau2`` (kx, ix, il, 4) etc.
     Field names are those of jcm/physics/speedy/physics_data.py."""
 
-    def __init__(self, diag: torch.Tensor, names, kx: int, hshape):
+    def __init__(self, diag: torch.Tensor, names, kx: int, hshape, date: DateData | None = None):
         self._diag = diag          # (rows, ncol) or (nmember, rows, ncol)
         self._names = names
         self._kx = kx
         self._hshape = tuple(hshape)
         self._index = {n: i for i, n in enumerate(names)}
+        self.date = date           # PhysicsData.date (physics_data.py), read by the term calls
 
     def _row(self, name):
         i = self._index[name]
@@ -103,6 +105,45 @@ class PhysicsData:
         for k, v in sub.items():
             ns[k] = torch.stack([v[i] for i in sorted(v)], dim=-1) if isinstance(v, dict) else v
         return SimpleNamespace(**ns)
+
+
+class SpeedyTerm:
+    """One SPEEDY term with the reference's call shape,
+    ``term(state, physics_data, parameters, forcing, geometry) -> (PhysicsTendency,
+    PhysicsData)`` (speedy_physics.py:123-125), on the GPU.
+
+    The kernels run the term chain fused, so a call runs the chain on ``state``
+    with ``parameters``, ``forcing`` and ``geometry`` at the date of
+    ``physics_data`` (``physics_data.date``, or the ``date`` keyword) and
+    returns:
+      * this term's own tendency: the value the full compute_tendencies adds for
+        it (term_mask = 1 << index), bit for bit;
+      * the PhysicsData the chain writes (the library's diagnostic subset), which
+        holds every term's outputs, those of the terms after this one included.
+    The fields of the incoming ``physics_data`` are not read: the terms before
+    this one are recomputed from ``state``, which equals the reference's result
+    whenever ``physics_data`` came from those terms on the same inputs (the way
+    compute_tendencies chains them)."""
+
+    def __init__(self, physics: "SpeedyPhysics", index: int, name: str):
+        self.physics, self.index, self.__name__ = physics, index, name
+
+    def __repr__(self):
+        return f"SpeedyTerm({self.index}, {self.__name__})"
+
+    def __call__(self, state: PhysicsState, physics_data, parameters: Parameters, forcing: ForcingData,
+                 geometry: Geometry, date: DateData | None = None):
+        date = date if date is not None else getattr(physics_data, "date", None)
+        if date is None:
+            raise ValueError(f"{self.__name__}: pass physics_data with a date (get_empty_data(geometry, date)) "
+                             "or the date keyword")
+        # a shallow copy with these parameters and diagnostics on: it shares the
+        # workspaces and caches of the physics object, which stays unchanged
+        q = copy.copy(self.physics)
+        q.with_diagnostics = True
+        if parameters is not None and parameters is not q.parameters:
+            q.parameters, q._abi_params, q._abi_params_src = parameters, None, None
+        return q.compute_tendencies(state, forcing, geometry, date, term_mask=1 << self.index)
 
 
 class DiagnosticsCollector:
@@ -352,8 +393,15 @@ class SpeedyPhysics(Physics):
         _native.check(rc, "speedy_fwd")
         if self.check_barrier:
             self._raise_on_barrier(ix, il, nmember, stream)
-        data = PhysicsData(diag, _native.diag_names(kx), kx, (ix, il)) if diag is not None else None
+        data = PhysicsData(diag, _native.diag_names(kx), kx, (ix, il), date) if diag is not None else None
         return PhysicsTendency(*out), data
+
+    @property
+    def terms(self) -> list:
+        """The eleven terms in order (speedy_physics.py:67-80), each callable as the
+        reference's ``term(state, physics_data, parameters, forcing, geometry)``
+        (speedy_physics.py:123-125); see SpeedyTerm."""
+        return [SpeedyTerm(self, i, name) for i, name in enumerate(_native.TERMS)]
 
     def prepare_vjp(self, state: PhysicsState, forcing: ForcingData, geometry: Geometry, date: DateData,
                     cotangent: PhysicsTendency, keep: list):
@@ -444,9 +492,10 @@ class SpeedyPhysics(Physics):
                       "speedy_workspace_status")
         return int(st.value)
 
-    def get_empty_data(self, geometry: Geometry):
-        """speedy_physics.py:129-133: an all-zero data struct for accumulating averages."""
+    def get_empty_data(self, geometry: Geometry, date: DateData | None = None):
+        """speedy_physics.py:129-133: an all-zero data struct for accumulating averages
+        (``date`` as PhysicsData.zeros takes it)."""
         kx, ix, il = geometry.nodal_shape
         rows = self._lib.speedy_diag_rows(kx)
         z = torch.zeros((rows, ix * il), dtype=torch.float32, device=self.device)
-        return PhysicsData(z, _native.diag_names(kx), kx, (ix, il))
+        return PhysicsData(z, _native.diag_names(kx), kx, (ix, il), date)
