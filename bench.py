@@ -89,6 +89,8 @@ def parse():
     ap.add_argument("--profile-only", action="store_true",
                     help="replay --steps forward and fwd+VJP calls, run a copy of --copy-mib MiB, and exit")
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--transforms-only", action="store_true",
+                    help="time only the T85 dynamics <-> physics transform step (--steps steps) and print its JSON")
     return ap.parse_args()
 
 
@@ -421,6 +423,9 @@ def main():
     from jcm_amd.speedy_physics import SpeedyPhysics
 
     dev = torch.device("cuda", 0)
+    if args.transforms_only:
+        print(json.dumps(transforms_leg(dev, reps=max(1, args.steps // 20))), flush=True)
+        return
     case_np = synthetic.make_case(256, 128, 8, seed=1, land=True)
     case = to_device(case_np, dev)
     kx, ix, il = case_np[2].nodal_shape
