@@ -242,15 +242,18 @@ def serving_runs(phys, dev, n_requests, rates, seed=7):
         cot = PhysicsTendency(*[torch.randn(st.temperature.shape, device=dev) for _ in range(4)])
         pool.append((st, fo, ge, da, cot))
     out = {"workload": f"{n_requests} requests, resolution uniform over T21/T31/T42/T85 x 8 levels, "
-                       "50/50 forward and forward+VJP, Poisson arrivals, 2 streams, 4 GiB HBM budget; "
-                       "measured after one untimed pass of the same stream",
+                       "50/50 forward and forward+VJP, Poisson arrivals (and all at once: saturation), 2 streams, "
+                       "4 GiB HBM budget; measured after one untimed pass of the same stream",
            "runs": []}
-    for rate in rates:
+    # the Poisson streams, then saturation (rate None: every request queued at once,
+    # realtime=False), where throughput is no longer capped by the offered load
+    for rate in list(rates) + [None]:
         rng = np.random.default_rng(seed)
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+        arrivals = np.cumsum(rng.exponential(1.0 / (rate or 1e9), n_requests))
         picks = rng.integers(0, len(res), n_requests)
         kinds = rng.integers(0, 2, n_requests)
-        run = {"rate_per_s": rate}
+        realtime = rate is not None
+        run = {"rate_per_s": rate if realtime else "saturation"}
         for mode, batching in (("batched", True), ("per_request", False)):
             def stream():
                 reqs = []
@@ -262,13 +265,15 @@ def serving_runs(phys, dev, n_requests, rates, seed=7):
             server = PhysicsServer(phys, hbm_budget=4 << 30, n_streams=2, batching=batching)
             # warm-up: the same stream once, untimed, so that workspaces, batch
             # descriptors and member structs exist before the measured run
-            server.run(stream(), realtime=True, timeout=60.0)
+            server.run(stream(), realtime=realtime, timeout=60.0)
             torch.cuda.synchronize()
             server.reset_stats()
             reqs = stream()
-            server.run(reqs, realtime=True, timeout=60.0)
+            server.run(reqs, realtime=realtime, timeout=60.0)
             torch.cuda.synchronize()
             run[mode] = server.stats(reqs)
+        b, p = run["batched"].get("columns_per_s"), run["per_request"].get("columns_per_s")
+        run["throughput_batched_over_per_request"] = b / p if b and p else None
         out["runs"].append(run)
     return out
 

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import operator
 import time
 import weakref
 from collections import deque
@@ -60,13 +61,12 @@ class Request:
     arrival: float = 0.0          # seconds on the server clock
     rid: int = 0
     # filled by the server
-    tendency: Optional[PhysicsTendency] = None
-    state_grads: Optional[PhysicsState] = None
-    param_grads: Optional[dict] = None
     level_set: Optional[list] = None   # per member, when the server records level sets
     t_admit: Optional[float] = None
     t_done: Optional[float] = None
     batch: int = -1
+    # (tendency, state_grads, param_grads), or the batch output block they are views of
+    _results: Any = field(default=None, repr=False)
 
     @property
     def kind(self) -> str:
@@ -74,14 +74,65 @@ class Request:
 
     @property
     def columns(self) -> int:
-        kx, ix, il = self.geometry.nodal_shape
-        t = self.state.temperature
-        nmember = 1 if len(t.shape) == 3 else int(np.prod(t.shape[:-3]))
-        return ix * il * nmember
+        c = self.__dict__.get("_ncols")
+        if c is None:
+            kx, ix, il = self.geometry.nodal_shape
+            t = self.state.temperature
+            nmember = 1 if len(t.shape) == 3 else int(np.prod(t.shape[:-3]))
+            c = self._ncols = ix * il * nmember
+        return c
+
+    def _get(self, i):
+        r = self._results
+        if isinstance(r, _Outputs):
+            self._results = r = r.views()
+        return None if r is None else r[i]
+
+    def _set(self, i, v):
+        self._get(0)   # a lazy block becomes its views first
+        r = list(self._results) if self._results is not None else [None, None, None]
+        r[i] = v
+        self._results = tuple(r)
 
     @property
     def latency(self) -> Optional[float]:
         return None if self.t_done is None else self.t_done - self.arrival
+
+    # the outputs: views of the batch's output block, made when first read
+    tendency = property(lambda self: self._get(0), lambda self, v: self._set(0, v))
+    state_grads = property(lambda self: self._get(1), lambda self, v: self._set(1, v))
+    param_grads = property(lambda self: self._get(2), lambda self, v: self._set(2, v))
+
+
+class _Outputs:
+    """One request's outputs inside a batch's output block (float32, device):
+    du, dv, dT, dq [, dL/du, dL/dv, dL/dT, dL/dq] of the state's shape, then for
+    the VJP dL/dps and the five parameter sums per member."""
+    __slots__ = ("buf", "off", "tshape", "vjp")
+
+    def __init__(self, buf, off, tshape, vjp):
+        self.buf, self.off, self.tshape, self.vjp = buf, off, tshape, vjp
+
+    def views(self):
+        tshape, n3 = self.tshape, math.prod(self.tshape)
+        lead, (ix, il) = tshape[:-3], tshape[-2:]
+        nout = 8 if self.vjp else 4
+        blk = self.buf[self.off:self.off + nout * n3].view((nout,) + tshape).unbind(0)
+        tend = PhysicsTendency(*blk[:4])
+        if not self.vjp:
+            return tend, None, None
+        n2 = n3 // tshape[-3]
+        o = self.off + 8 * n3
+        g_ps = self.buf[o:o + n2].view(lead + (ix, il))
+        g_par = self.buf[o + n2:o + n2 + 5 * (n2 // (ix * il))].view(lead + (5,))
+        return tend, PhysicsState(blk[4], blk[5], blk[6], blk[7], None, g_ps), dict(zip(_VJP_NAMES, g_par.unbind(-1)))
+
+
+def _output_floats(tshape, vjp: bool) -> int:
+    """Floats of one request's outputs (see _Outputs), rounded up to 64 (256 bytes)."""
+    n3 = math.prod(tshape)
+    n = (8 * n3 + n3 // tshape[-3] + 5 * (n3 // (tshape[-3] * tshape[-2] * tshape[-1]))) if vjp else 4 * n3
+    return (n + 63) // 64 * 64
 
 
 def request_footprint(req: Request, kx: int, workspace_bytes: int) -> int:
@@ -220,45 +271,43 @@ class PhysicsServer:
             members[key] = hit
         return hit
 
-    def _prepare(self, req: Request, seg, stream):
-        """Fill one batch segment for req; allocates its outputs in one block."""
+    def _prepare(self, req: Request, seg, buf: torch.Tensor, off: int):
+        """Fill one batch segment for req, its outputs at float offset `off` of the
+        batch's output block (views are made when the client first reads them)."""
         f_in, tshape, (ix, il, nm), flags, member_keep = self._member(req)[:5]
         seg.fields = f_in
         f = seg.fields                     # a view of the segment's memory
         n3 = math.prod(tshape)
-        lead = tshape[:-3]
         vjp = req.cotangent is not None
-        n2 = nm * ix * il
-        nout = 8 if vjp else 4
-        total = nout * n3 + ((n2 + nm * 5) if vjp else 0)
-        # allocated on the caller's stream, where the results are read; the batch
-        # stream's use is recorded so the block is not reused before it is done
-        buf = torch.empty(total, dtype=torch.float32, device=self.dev)
-        buf.record_stream(stream)
-        views = buf[:nout * n3].view((nout,) + tshape).unbind(0)
-        base, step = buf.data_ptr(), 4 * n3
+        base, step = buf.data_ptr() + 4 * off, 4 * n3
         f.du, f.dv, f.dt, f.dq = base, base + step, base + 2 * step, base + 3 * step
-        req.tendency = PhysicsTendency(*views[:4])
         if vjp:
             adj = seg.adj
-            adj.ct_du, adj.ct_dv, adj.ct_dt, adj.ct_dq = self._cotangent_ptrs(req.cotangent, tshape)
-            g_ps = buf[8 * n3:8 * n3 + n2].view(lead + (ix, il))
-            g_par = buf[8 * n3 + n2:].view(lead + (5,))
+            adj.ct_du, adj.ct_dv, adj.ct_dt, adj.ct_dq = self._cotangent_ptrs(req.cotangent, tshape, req)
+            n2 = nm * ix * il
             adj.g_u, adj.g_v, adj.g_t, adj.g_q = base + 4 * step, base + 5 * step, base + 6 * step, base + 7 * step
             adj.g_ps, adj.g_params = base + 8 * step, base + 8 * step + 4 * n2
-            req.state_grads = PhysicsState(views[4], views[5], views[6], views[7], None, g_ps)
-            req.param_grads = dict(zip(_VJP_NAMES, g_par.unbind(-1)))
-        return (ix, il, nm), flags, buf, member_keep
+        req._results = _Outputs(buf, off, tshape, vjp)
+        return (ix, il, nm), flags, member_keep
 
-    def _cotangent_ptrs(self, ct: PhysicsTendency, tshape) -> tuple:
-        """Device addresses of a cotangent's four fields, after checking them.
-        Cotangents are usually new with every request, so nothing is cached."""
+    def _cotangent_ptrs(self, ct: PhysicsTendency, tshape, req: Request | None = None) -> tuple:
+        """Device addresses of a cotangent's four fields, after checking them.  The
+        check made at submit is kept on the request and reused at launch while the
+        cotangent still holds the same tensors at the same addresses."""
         cts = (ct.u_wind, ct.v_wind, ct.temperature, ct.specific_humidity)
+        hit = getattr(req, "_ct", None) if req is not None else None
+        if hit is not None and all(map(operator.is_, hit[0], cts)):
+            ptrs = tuple(c.data_ptr() for c in cts)
+            if ptrs == hit[1]:
+                return ptrs
         for c in cts:
             if not (isinstance(c, torch.Tensor) and c.dtype is torch.float32 and c.shape == tshape
                     and c.get_device() == self._dev_index and c.is_contiguous()):
                 raise ValueError("cotangents must be contiguous float32 device tensors of the state's shape")
-        return tuple(c.data_ptr() for c in cts)
+        ptrs = tuple(c.data_ptr() for c in cts)
+        if req is not None:
+            req._ct = (cts, ptrs)
+        return ptrs
 
     # ------------------------------------------------------------ loop
     def submit(self, req: Request):
@@ -271,7 +320,7 @@ class PhysicsServer:
         if shape[-3:] != (kx, ix, il):
             raise ValueError(f"request {req.rid}: state shape {shape} does not match geometry {(kx, ix, il)}")
         if req.cotangent is not None:
-            self._cotangent_ptrs(req.cotangent, shape)
+            self._cotangent_ptrs(req.cotangent, shape, req)
         self.queue.append(req)
 
     def _free_slot(self) -> Optional[_Slot]:
@@ -307,10 +356,19 @@ class PhysicsServer:
         s_ptr = ctypes.c_void_p(slot.stream.cuda_stream)
         # the states were written on the client's stream
         self._wait_client(slot)
+        # one output block for the whole batch, allocated on the caller's stream (where
+        # the results are read); the batch stream's use is recorded so the block is not
+        # reused before the batch is done
+        offs, total = [], 0
+        for req in batch:
+            offs.append(total)
+            total += _output_floats(tuple(req.state.temperature.shape), req.cotangent is not None)
+        buf = torch.empty(total, dtype=torch.float32, device=self.dev)
+        buf.record_stream(slot.stream)
+        keep.append(buf)
         for i, req in enumerate(batch):
             seg = segs[i]
-            (ix, il, nm), flags, buf, member_keep = self._prepare(req, seg, slot.stream)
-            keep.append(buf)
+            (ix, il, nm), flags, member_keep = self._prepare(req, seg, buf, offs[i])
             keep.append(member_keep)   # device copies of a NumPy state live until the batch completes
             ws = self._workspace((ix, il, nm), slot.stream)
             req._ws = ((ix, il, nm), ws)
