@@ -135,9 +135,16 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
   const int m0 = (blockIdx.x / tiles_n) * kBM, n0 = (blockIdx.x % tiles_n) * kBN;
 
   // two LDS stages: the global loads of K-step s+1 are in flight (in registers)
-  // while the MFMAs of step s read the other stage; one barrier per step
-  __shared__ float As[2][kBK][kBM + kPad];
-  __shared__ float Bs[2][kBK][kBN + kPad];
+  // while the MFMAs of step s read the other stage; one barrier per step.  Tiles
+  // are stored [row][k] (row stride kLdk floats).  Within a step the k order is
+  // permuted: lane group g = lane / 16 takes k = 8 g + q at substep q, for A and B
+  // alike, so each MFMA still pairs equal k and a lane's eight operands of a step
+  // are consecutive: two 16-byte LDS reads per operand row, conflict-free at this
+  // stride (16 lanes x 4 banks cover all banks once).
+  static_assert(kBK == 32, "the k permutation assumes 32-wide K steps");
+  constexpr int kLdk = kBK + kPad;
+  __shared__ __align__(16) float As[2][kBM][kLdk];
+  __shared__ __align__(16) float Bs[2][kBN][kLdk];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   floatx4 acc[2][2];
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
       const int e = t + kThreads * r;
       const int am = kAKContig ? (e / kBK) : (e % kBM);
       const int ak = kAKContig ? (e % kBK) : (e / kBM);
-      As[st][ak][am] = (m0 + am < M && k0 + ak < K) ? ra[r] : 0.0f;
+      As[st][am][ak] = (m0 + am < M && k0 + ak < K) ? ra[r] : 0.0f;
       const int nn = kBNContig ? (e % kBN) : (e / kBK);
       const int kk = kBNContig ? (e / kBN) : (e % kBK);
       float v = rb[r];
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
         if (sg.op & kOpSubLevel) v -= lvl;
         v *= colscale;
       }
-      Bs[st][kk][nn] = (k0 + kk < K && n0 + nn < N) ? v : 0.0f;
+      Bs[st][nn][kk] = (k0 + kk < K && n0 + nn < N) ? v : 0.0f;
     }
   };
   load(0);
@@ -199,25 +206,26 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
     const bool more = k0 + kBK < K;
     if (more) load(k0 + kBK);
     // all operands of this step out of LDS first, then the MFMAs
-    float av[kBK / 4][2], bv[kBK / 4][2];
+    const int g8 = (lane >> 4) * 8, r16 = lane & 15;
+    float4 av[2][2], bv[2][2];
 #pragma unroll
-    for (int q = 0; q < kBK / 4; ++q) {
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) av[q][i] = As[st][4 * q + (lane >> 4)][wm + 16 * i + (lane & 15)];
+      for (int i = 0; i < 2; ++i) av[i][h] = *reinterpret_cast<const float4*>(&As[st][wm + 16 * i + r16][g8 + 4 * h]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bv[q][j] = Bs[st][4 * q + (lane >> 4)][wn + 16 * j + (lane & 15)];
+      for (int j = 0; j < 2; ++j) bv[j][h] = *reinterpret_cast<const float4*>(&Bs[st][wn + 16 * j + r16][g8 + 4 * h]);
     }
 #pragma unroll
-    for (int q = 0; q < kBK / 4; ++q)
+    for (int q = 0; q < 8; ++q)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][i], bv[q][j], acc[i][j], 0, 0, 0);
-    // keep the LDS reads ahead of the MFMAs (the default schedule waits on each
-    // substep's read right before its four MFMAs)
-    __builtin_amdgcn_sched_group_barrier(0x100, kBK / 2, 0);
-    __builtin_amdgcn_sched_group_barrier(0x008, kBK, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i][q >> 2][q & 3], bv[j][q >> 2][q & 3], acc[i][j],
+                                                           0, 0, 0);
+    // keep the LDS reads ahead of the MFMAs
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
     if (more) store(st ^ 1, k0 + kBK);
     __syncthreads();
     st ^= 1;
