@@ -204,3 +204,42 @@ def test_bad_shapes_raise(tr):
     z = np.zeros((7,) + og.modal_shape, np.float32)
     with pytest.raises(ValueError):
         tr.dynamics_state_to_physics_state(tr.DynamicsState(z, z, z, z[:1], {"specific_humidity": z}), dyn)
+
+
+def test_graph_replays_equal_eager_calls(tr):
+    """The conversions captured in a HIP graph: several replays, with eager work and
+    synchronisations between them, each equal to the eager result bit for bit."""
+    import torch
+    from jcm_amd.physics_interface import PhysicsTendency
+    gg, og = _grids(tr, 42)
+    kx = 8
+    rng = np.random.default_rng(9)
+    dyn, _ = _dynamics(tr, gg, og, kx, rng)
+    dev = torch.device("cuda", 0)
+    dyn.reference_temperature = torch.as_tensor(np.asarray(dyn.reference_temperature), device=dev)
+    dyn.orography = torch.as_tensor(np.asarray(dyn.orography), device=dev)
+    modal = lambda lead, sc: torch.as_tensor(_random_modal(og, lead, rng, sc).astype(np.float32), device=dev)
+    st = tr.DynamicsState(modal((kx,), 1e-5), modal((kx,), 1e-6), modal((kx,), 5.0), modal((1,), 0.02),
+                          {"specific_humidity": modal((kx,), 1e-3)})
+    shape = (kx,) + og.nodal_shape
+    tend = PhysicsTendency(*(torch.randn(shape, device=dev) * s for s in (1e-4, 1e-4, 1e-5, 1e-6)))
+    fn = lambda: (tr.dynamics_state_to_physics_state(st, dyn), tr.physics_tendency_to_dynamics_tendency(tend, dyn))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        ref = fn()                       # eager, on the stream the graph is captured on
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        out = fn()
+    for _ in range(3):
+        graph.replay()
+        torch.cuda.synchronize()
+        _ = torch.randn(1 << 20, device=dev).sum()   # eager work between replays
+        torch.cuda.synchronize()
+        for name in ("u_wind", "v_wind", "temperature", "specific_humidity", "geopotential",
+                     "normalized_surface_pressure"):
+            assert torch.equal(getattr(out[0], name), getattr(ref[0], name)), name
+        for name in ("vorticity", "divergence", "temperature_variation", "log_surface_pressure"):
+            assert torch.equal(getattr(out[1], name), getattr(ref[1], name)), name
+        assert torch.equal(out[1].tracers["specific_humidity"], ref[1].tracers["specific_humidity"])
