@@ -57,7 +57,7 @@ def _dynamics(tr, gg, og, kx, rng):
     return dyn_gpu, dyn_ref
 
 
-@pytest.mark.parametrize("trunc", [21, 31, 85])
+@pytest.mark.parametrize("trunc", [21, 31, 85, 170])
 def test_to_nodal_and_to_modal_match_the_oracle(tr, trunc):
     import torch
     gg, og = _grids(tr, trunc)
