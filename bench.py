@@ -360,12 +360,13 @@ MFMA_FP32_PEAK_TFS = 157.3
 
 def transform_work(g, kx):
     """(flops, bytes) of one transform step: dynamics -> physics state plus physics
-    tendency -> dynamics at the grid g.  Flops: 2 per multiply-add of the Legendre
-    and Fourier contractions (5*kx+1 fields synthesised: winds, temperature,
+    tendency -> dynamics at the grid g.  Flops: 2 per multiply-add of the Fourier
+    contraction and of the Legendre contraction as computed, over half the latitudes
+    by the equatorial symmetry (5*kx+1 fields synthesised: winds, temperature,
     humidity, log surface pressure, geopotential; 4*kx analysed).  Bytes: modal
     inputs and outputs, nodal inputs and outputs, and each basis read once."""
     mm, nl, nlon, nlat = 2 * g.n_m - 1, g.n_l, g.nlon, g.nlat
-    contraction = lambda nf: 2.0 * nf * mm * nlat * (nl + nlon)
+    contraction = lambda nf: 2.0 * nf * mm * nlat * (nl / 2.0 + nlon)
     flops = contraction(5 * kx + 1) + contraction(4 * kx)
     modal, nodal = mm * nl * 4, nlon * nlat * 4
     bases = g.n_m * nlat * nl * 4 + nlon * mm * 4

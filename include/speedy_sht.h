@@ -57,7 +57,8 @@ int speedy_sht_host_basis(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, 
 /* A plan holds the device bases of one grid (Grid.T<N>: n_m longitude
  * wavenumbers, n_l total wavenumbers, nlon x nlat nodes, sphere radius in m).
  * Creation allocates and uploads them on the current device (synchronous);
- * it requires n_m <= n_l, 2*n_m-1 <= nlon and n_l <= nlat. */
+ * it requires n_m <= n_l, 2 <= n_l <= nlat, 2*n_m-1 <= nlon and an even nlat
+ * (the Legendre stages use the equatorial symmetry of the Gaussian grid). */
 int speedy_sht_create(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, double radius,
                       SpeedyShtPlan **plan);
 int speedy_sht_destroy(SpeedyShtPlan *plan);

@@ -91,13 +91,25 @@ __device__ __forceinline__ int find_seg(const SegTable& t, int f) {
 // (r, c) of X at X[r * x_r + c * x_c].
 template <int STAGE>
 __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
-  const int b = blockIdx.y;
+  // Legendre stages: blockIdx.y = 2 * mode + parity.  Gaussian latitudes are
+  // symmetric (x_{nlat-1-j} = -x_j) and P_l^m(-x) = (-1)^(l+m) P_l^m(x), so each
+  // mode splits into the l with l+m even (parity 0) and odd (parity 1), each over
+  // the southern half of the latitudes: half the work and half the K of a full
+  // contraction.  Analysis reads Y_j +- Y_{nlat-1-j}; synthesis writes the even
+  // and odd partial sums E, O for the southern half ([parity][field][mode][h]),
+  // which the Fourier synthesis combines as E + O (south) and E - O (north).
+  const bool leg = STAGE == kLegendreAnalysis || STAGE == kLegendreSynthesis;
+  const int b = leg ? (int)(blockIdx.y >> 1) : (int)blockIdx.y;
+  const int par = leg ? (int)(blockIdx.y & 1u) : 0;
+  const int h = a.nlat >> 1;
   int M, N, K;
   const float* A;
   const float* B;
   float* C;
-  long a_m, a_k, b_k, b_n, c_m;
+  long a_m, a_k, b_k, b_n, c_m, c_n = 1;
   const int mode_m = (b + 1) >> 1;    // Legendre stages: order m of mode b
+  const int lofs = (mode_m + par) & 1;   // first l of this parity (l = lofs + 2c)
+  const int npar = (a.nl - lofs + 1) >> 1;
   Seg sg{};
   float lvl = 0.0f;
   if (STAGE == kFourierAnalysis) {       // Y_b[mm x nlat] = F^T[mm x nlon] * x_b[nlon x nlat]
@@ -107,24 +119,29 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
     B = sg.src + (size_t)(b - sg.first) * a.nlon * a.nlat; b_k = a.nlat; b_n = 1;
     C = a.out + (size_t)b * a.mm * a.nlat; c_m = a.nlat;
     if (sg.level) lvl = sg.level[(b - sg.first) % sg.period];
-  } else if (STAGE == kLegendreAnalysis) {   // Z[:, b, :] [nfield x L] = Y[:, b, :] [nfield x nlat] * P_m [nlat x L]
-    M = a.nfield; N = a.nl; K = a.nlat;
+  } else if (STAGE == kLegendreAnalysis) {
+    // Z[:, b, lofs + 2c] [nfield x npar] = (Y_j +- Y_{nlat-1-j})[:, b, :h] [nfield x h] * P_m[:h, lofs + 2c]
+    M = a.nfield; N = npar; K = h;
     A = a.in + (size_t)b * a.nlat; a_m = (long)a.mm * a.nlat; a_k = 1;
-    B = a.P + (size_t)mode_m * a.nlat * a.nl; b_k = a.nl; b_n = 1;
-    C = a.out + (size_t)b * a.nl; c_m = (long)a.mm * a.nl;
-  } else if (STAGE == kLegendreSynthesis) {  // Y[:, b, :] [nfield x nlat] = X[:, b, :] [nfield x L] * P_m^T [L x nlat]
-    M = a.nfield; N = a.nlat; K = a.nl;
-    A = a.in + (size_t)b * a.nl; a_m = (long)a.mm * a.nl; a_k = 1;
-    B = a.P + (size_t)mode_m * a.nlat * a.nl; b_k = 1; b_n = a.nl;
-    C = a.out + (size_t)b * a.nlat; c_m = (long)a.mm * a.nlat;
+    B = a.P + (size_t)mode_m * a.nlat * a.nl + lofs; b_k = a.nl; b_n = 2;
+    C = a.out + (size_t)b * a.nl + lofs; c_m = (long)a.mm * a.nl; c_n = 2;
+  } else if (STAGE == kLegendreSynthesis) {
+    // E|O[par][:, b, :h] [nfield x h] = X[:, b, lofs + 2c] [nfield x npar] * P_m[:h, lofs + 2c]^T
+    M = a.nfield; N = h; K = npar;
+    A = a.in + (size_t)b * a.nl + lofs; a_m = (long)a.mm * a.nl; a_k = 2;
+    B = a.P + (size_t)mode_m * a.nlat * a.nl + lofs; b_k = 2; b_n = a.nl;
+    C = a.out + (size_t)par * a.nfield * a.mm * h + (size_t)b * h; c_m = (long)a.mm * h;
   } else {                                   // out_b [nlon x nlat] = F [nlon x mm] * Y_b [mm x nlat]
     sg = a.seg.s[find_seg(a.seg, b)];
     M = a.nlon; N = a.nlat; K = a.mm;
     A = a.F; a_m = a.mm; a_k = 1;
-    B = a.in + (size_t)b * a.mm * a.nlat; b_k = a.nlat; b_n = 1;
+    // Y_b from the even / odd halves: [par][field][mode][h]
+    B = a.in + (size_t)b * a.mm * h; b_k = h; b_n = 1;
     C = sg.dst + (size_t)(b - sg.first) * a.nlon * a.nlat; c_m = a.nlat;
     if (sg.level) lvl = sg.level[(b - sg.first) % sg.period];
   }
+  const float sgn = par ? -1.0f : 1.0f;
+  const size_t odd_off = (size_t)a.nfield * a.mm * h;   // Fourier synthesis: O after E
   constexpr bool kAKContig = STAGE != kFourierAnalysis;
   constexpr bool kBNContig = STAGE != kLegendreSynthesis;
   constexpr int kLoads = kBM * kBK / kThreads;
@@ -170,14 +187,26 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
       const int kk = kAKContig ? (e % kBK) : (e / kBM);
       // clamped indices (always in bounds), no condition: the loads issue back to
       // back; elements outside the matrix are zeroed when they go to LDS
-      ra[r] = A[min(m0 + mm, M - 1) * a_m + min(k0 + kk, K - 1) * a_k];
+      const long row = min(m0 + mm, M - 1) * a_m;
+      const int ck = min(k0 + kk, K - 1);
+      if (STAGE == kLegendreAnalysis)
+        ra[r] = A[row + ck] + sgn * A[row + (a.nlat - 1 - ck)];
+      else
+        ra[r] = A[row + ck * a_k];
     }
 #pragma unroll
     for (int r = 0; r < kLoads; ++r) {
       const int e = t + kThreads * r;
       const int nn = kBNContig ? (e % kBN) : (e / kBK);
       const int kk = kBNContig ? (e / kBN) : (e % kBK);
-      rb[r] = B[min(k0 + kk, K - 1) * b_k + min(n0 + nn, N - 1) * b_n];
+      const int ck = min(k0 + kk, K - 1), cn = min(n0 + nn, N - 1);
+      if (STAGE == kFourierSynthesis) {   // latitude cn from E + O (south) or E - O (north)
+        const int j = cn < h ? cn : a.nlat - 1 - cn;
+        const float* e = B + (size_t)ck * h + j;
+        rb[r] = cn < h ? e[0] + e[odd_off] : e[0] - e[odd_off];
+      } else {
+        rb[r] = B[ck * b_k + cn * b_n];
+      }
     }
   };
   auto store = [&](int st, int k0) {
@@ -248,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
           if (sg.op & kOpAddLevel) v += lvl;
           if (sg.op & kOpExp) v = expf(v);
         }
-        C[gm * c_m + gn] = v;
+        C[gm * c_m + gn * c_n] = v;
       }
     }
 }
@@ -430,9 +459,10 @@ void gauss_nodes(int n, double* x, double* w) {
 }
 
 int check_grid(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat) {
-  if (n_m < 1 || n_l < 1 || nlon < 1 || nlat < 2 || n_m > n_l || 2 * n_m - 1 > nlon || n_l > nlat ||
+  if (n_m < 1 || n_l < 2 || nlon < 1 || nlat < 2 || (nlat & 1) || n_m > n_l || 2 * n_m - 1 > nlon || n_l > nlat ||
       (int64_t)n_m * nlat * n_l >= ((int64_t)1 << 31) || (int64_t)nlon * nlat * 64 >= ((int64_t)1 << 31))
-    return fail(SPEEDY_ERR_ARG, "speedy_sht: bad grid (need 1 <= n_m <= n_l <= nlat, 2*n_m-1 <= nlon)");
+    return fail(SPEEDY_ERR_ARG, "speedy_sht: bad grid (need 1 <= n_m <= n_l <= nlat, n_l >= 2, 2*n_m-1 <= nlon, "
+                                "nlat even)");
   return SPEEDY_OK;
 }
 
@@ -507,16 +537,16 @@ hipError_t run_analysis(const SpeedyShtPlan* p, StageArgs a, float* y, float* mo
   if (e != hipSuccess) return e;
   a.in = y;
   a.out = modal_out;
-  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreAnalysis>, dim3(tiles(a.nfield, p->n_l), p->mm), dim3(kThreads), 0, s,
-                     a);
+  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreAnalysis>, dim3(tiles(a.nfield, (p->n_l + 1) / 2), 2 * p->mm),
+                     dim3(kThreads), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t run_synthesis(const SpeedyShtPlan* p, StageArgs a, const float* modal_in, float* y, hipStream_t s) {
   a.in = modal_in;
   a.out = y;
-  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreSynthesis>, dim3(tiles(a.nfield, p->nlat), p->mm), dim3(kThreads), 0,
-                     s, a);
+  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreSynthesis>, dim3(tiles(a.nfield, p->nlat / 2), 2 * p->mm),
+                     dim3(kThreads), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   a.in = y;
