@@ -36,8 +36,11 @@ Every function is pinned from the mathematics in tests/test_sht_oracle.py:
 Gauss quadrature exactness, scipy's associated Legendre functions, basis
 orthonormality, closed-form winds of solid-body rotation and of low-degree
 harmonics, isothermal and linear-in-log(sigma) geopotentials, and the round
-trip of physics_interface_test.py:9-44.  Only tests/, smoke() and bench.py's
-CPU baseline use this module; the library never does.
+trip of physics_interface_test.py:9-44.  Parity unpinned: ``clip`` (the
+highest total wavenumber zeroed after each gradient, curl and divergence) is
+dinosaur's truncation convention, restated; the closed-form fields have no
+energy there, so no test can tell it from its absence.  Only tests/, smoke()
+and bench.py's CPU baseline use this module; the library never does.
 """
 from __future__ import annotations
 
