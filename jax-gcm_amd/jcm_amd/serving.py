@@ -52,7 +52,10 @@ _STATE_NAMES = ("u_wind", "v_wind", "temperature", "specific_humidity", "geopote
 
 @dataclass
 class Request:
-    """One member's physics call.  Results are filled in when it completes."""
+    """One member's physics call.  Results are filled in when it completes: views
+    of the output block its batch shares with the batch's other requests, so a
+    result that is kept keeps that block allocated (copy it to keep one result
+    alone)."""
     state: PhysicsState
     forcing: Any
     geometry: Any
