@@ -1024,7 +1024,7 @@ SPD_HD void column_vjp(const ColumnIn<R, KX>& c_in, const PhaseA<R, KX>& a, cons
     bb.rlds = b.rlds;
     LwuTape<R, KX> lt;
     const R fh[4] = {b.flux_h[0], b.flux_h[1], R(0.0), R(0.0)};
-    lw_up<R, KX, false>(cl, p, v, u, bb, b.rlus2, b.tsfc, fh, lt);
+    lw_up<R, KX, false>(cl, a, p, v, u, bb, b.rlus2, b.tsfc, fh, lt);
 #pragma unroll
     for (int k = 0; k < KX; ++k) g.rps += ct.dt[k] * lt.dfl[k] * v.grdscp[k];
 #pragma unroll

@@ -1076,10 +1076,13 @@ struct LwuTape {
 };
 
 // longwave up (longwave_radiation.py:119-202); dfl enters holding the
-// longwave-down flux divergence.
+// longwave-down flux divergence.  Each level's transmissivities and emissions
+// are recomputed (lw_level, equal to lw_transmissivity's and lw_emissions'), so
+// the [KX][4] arrays of phase b1 are dead after longwave down and do not stay
+// live across the shortwave and the surface fluxes.
 template <typename R, int KX, bool DIAG>
-SPD_HD void lw_up(const ColumnIn<R, KX>& c, const SpeedyParams& p, const SpeedyVertical& v, const Uniforms<R>& u,
-                  PhaseB<R, KX>& b, R rlus2, R tsfc, const R flux_h[4], LwuTape<R, KX>& t) {
+SPD_HD void lw_up(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p, const SpeedyVertical& v,
+                  const Uniforms<R>& u, PhaseB<R, KX>& b, R rlus2, R tsfc, const R flux_h[4], LwuTape<R, KX>& t) {
   const R ps = c.ps;
   const R rps = R(1.0) / ps;
   const R stratc0 = c.stratz * ps, stratc1 = u.eps1 * ps;
@@ -1094,14 +1097,16 @@ SPD_HD void lw_up(const ColumnIn<R, KX>& c, const SpeedyParams& p, const SpeedyV
   for (int k = 0; k < KX; ++k) dfl[k] = b.dfl[k];
   dfl[KX - 1] = dfl[KX - 1] + p.epslw * rlus2;
   R Gnext = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
+  R st4a0_1 = R(0.0);
 #pragma unroll
   for (int k = KX - 1; k >= 1; --k) {
-    R fb[4];
-    radset4(c.T[k], u.omeps, fb);
+    LwLevel<R> L;
+    lw_level<R, KX>(k, c, a, p, v, u, L);
+    if (k == 1) st4a0_1 = L.st0;
 #pragma unroll
     for (int bnd = 0; bnd < 4; ++bnd) {
-      const R em = R(1.0) - b.tauL[k][bnd];
-      Fu[bnd] = b.tauL[k][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[k] - em * b.st4a1[k]));
+      const R em = R(1.0) - L.tau[bnd];
+      Fu[bnd] = L.tau[bnd] * Fu[bnd] + em * (L.fb[bnd] * (L.st0 - em * L.st1));
       t.Fu[k][bnd] = Fu[bnd];
     }
     const R Gk = ((Fu[0] + Fu[1]) + Fu[2]) + Fu[3];
@@ -1109,21 +1114,19 @@ SPD_HD void lw_up(const ColumnIn<R, KX>& c, const SpeedyParams& p, const SpeedyV
     Gnext = Gk;
   }
   R ft[4];
-  {
-    R fb[4];
-    radset4(c.T[0], u.omeps, fb);
+  LwLevel<R> L0;
+  lw_level<R, KX>(0, c, a, p, v, u, L0);
 #pragma unroll
-    for (int bnd = 0; bnd < 2; ++bnd) {
-      const R em = R(1.0) - b.tauL[0][bnd];
-      ft[bnd] = b.tauL[0][bnd] * Fu[bnd] + em * (fb[bnd] * (b.st4a0[0] - em * b.st4a1[0]));
-    }
-    ft[2] = Fu[2];
-    ft[3] = Fu[3];
+  for (int bnd = 0; bnd < 2; ++bnd) {
+    const R em = R(1.0) - L0.tau[bnd];
+    ft[bnd] = L0.tau[bnd] * Fu[bnd] + em * (L0.fb[bnd] * (L0.st0 - em * L0.st1));
   }
+  ft[2] = Fu[2];
+  ft[3] = Fu[3];
   t.ft[0] = ft[0]; t.ft[1] = ft[1];
   dfl[0] = dfl[0] + ((Fu[0] - ft[0]) + (Fu[1] - ft[1]));
-  const R corlw1 = v.dhs[0] * stratc1 * b.st4a0[0] + stratc0;
-  const R corlw2 = v.dhs[1] * stratc1 * b.st4a0[1];
+  const R corlw1 = v.dhs[0] * stratc1 * L0.st0 + stratc0;
+  const R corlw2 = v.dhs[1] * stratc1 * st4a0_1;
   dfl[0] = dfl[0] + (-corlw1);
   dfl[1] = dfl[1] + (-corlw2);
 #pragma unroll
@@ -1150,7 +1153,7 @@ SPD_HD void phase_b2(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, uint32_t 
   surface<R, KX, DIAG>(c, a, p, v, u, flags, b, tsfc, sf);
   b.flux_h[0] = flux_h[0]; b.flux_h[1] = flux_h[1]; b.rlus2 = sf.rlus2; b.tsfc = tsfc;
   LwuTape<R, KX> lu;
-  lw_up<R, KX, DIAG>(c, p, v, u, b, sf.rlus2, tsfc, flux_h, lu);
+  lw_up<R, KX, DIAG>(c, a, p, v, u, b, sf.rlus2, tsfc, flux_h, lu);
 }
 
 }  // namespace speedy
