@@ -104,39 +104,54 @@ __device__ __forceinline__ uint32_t divq(uint32_t n, const Divisor& dv) {
   return q;
 }
 
+// Field access through address-space-1 (global) pointers.  The batch kernels
+// read their field pointers from a descriptor in memory, and the compiler emits
+// flat instructions for generic pointers; a flat access also counts in the
+// scalar-load counter, so every wait for a uniform's scalar load then waits for
+// the column's outstanding loads and stores too.  The kernel-argument kernels
+// get global instructions either way.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, size_t i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, size_t i, T v) {
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
 template <int KX, bool EXTRA>
 __device__ __forceinline__ void load_state(const KArgs& a, uint32_t m, uint32_t c, ColumnIn<float, KX>& col) {
   const size_t ncol = a.ncol;
   const size_t base3 = (size_t)m * KX * ncol + c;
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
-    col.T[k] = a.f.t[base3 + k * ncol];
-    col.Q[k] = a.f.q[base3 + k * ncol];
-    col.PHI[k] = a.f.phi[base3 + k * ncol];
+    col.T[k] = gld(a.f.t, base3 + k * ncol);
+    col.Q[k] = gld(a.f.q, base3 + k * ncol);
+    col.PHI[k] = gld(a.f.phi, base3 + k * ncol);
   }
   if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_STATE)) {
 #pragma unroll
     for (int k = 0; k < KX; ++k) col.Q[k] = verify_q(col.Q[k]);
   }
-  col.u1 = a.f.u[base3 + (KX - 1) * ncol];
-  col.v1 = a.f.v[base3 + (KX - 1) * ncol];
-  col.ps = a.f.ps[(size_t)m * ncol + c];
-  col.alb0 = a.f.alb0[c];
-  col.sice = a.f.sice_am[c];
-  col.snowc_am = a.f.snowc_am[c];
-  col.soilw = a.f.soilw_am[c];
-  col.stl = a.f.stl_am[c];
-  col.sst = a.f.sst[c];
-  col.fmask = a.f.fmask[c];
-  col.orog = a.f.orog[c];
-  col.phis0 = a.f.phis0[c];
+  col.u1 = gld(a.f.u, base3 + (KX - 1) * ncol);
+  col.v1 = gld(a.f.v, base3 + (KX - 1) * ncol);
+  col.ps = gld(a.f.ps, (size_t)m * ncol + c);
+  col.alb0 = gld(a.f.alb0, c);
+  col.sice = gld(a.f.sice_am, c);
+  col.snowc_am = gld(a.f.snowc_am, c);
+  col.soilw = gld(a.f.soilw_am, c);
+  col.stl = gld(a.f.stl_am, c);
+  col.sst = gld(a.f.sst, c);
+  col.fmask = gld(a.f.fmask, c);
+  col.orog = gld(a.f.orog, c);
+  col.phis0 = gld(a.f.phis0, c);
 }
 
 template <int KX>
 __device__ __forceinline__ void load_zonal(const KArgs& a, uint32_t c, ColumnIn<float, KX>& col) {
   const uint32_t j = c - divq(c, a.dlat) * a.nlat;
   float z[6];
-  zonal_column(a.u.zon, a.f.sia[j], a.f.coa[j], z);
+  zonal_column(a.u.zon, gld(a.f.sia, j), gld(a.f.coa, j), z);
   col.fsol = z[0]; col.ozupp = z[1]; col.ozone = z[2]; col.zenit = z[3]; col.stratz = z[4]; col.sqcoa = z[5];
 }
 
@@ -149,8 +164,8 @@ __device__ __forceinline__ void store_uv_zeros(const KArgs& a, uint32_t m, uint3
   const size_t base3 = (size_t)m * KX * ncol + c;
 #pragma unroll
   for (int k = 0; k < KX - 1; ++k) {
-    a.f.du[base3 + k * ncol] = 0.0f;
-    a.f.dv[base3 + k * ncol] = 0.0f;
+    gst(a.f.du, base3 + k * ncol, 0.0f);
+    gst(a.f.dv, base3 + k * ncol, 0.0f);
   }
 }
 
@@ -181,15 +196,15 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
     if (tm & (1u << SPEEDY_TERM_VDIFF)) { t = t + pb.tvd[k]; q = q + pb.qvd[k]; }
     if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_TENDENCIES)) {
       bool lim;
-      q = verify_dq(a.f.q[base3 + k * ncol], q, a.u.dt_step, lim);   // the unclamped q (re-read, L2)
+      q = verify_dq(gld(a.f.q, base3 + k * ncol), q, a.u.dt_step, lim);   // the unclamped q (re-read, L2)
       limited |= (lim ? 1u : 0u) << k;
     }
     if (UV_ALL || k == KX - 1) {   // without UV_ALL the levels above were zeroed by store_uv_zeros
-      a.f.du[base3 + k * ncol] = du;
-      a.f.dv[base3 + k * ncol] = dv;
+      gst(a.f.du, base3 + k * ncol, du);
+      gst(a.f.dv, base3 + k * ncol, dv);
     }
-    a.f.dt[base3 + k * ncol] = t;
-    a.f.dq[base3 + k * ncol] = q;
+    gst(a.f.dt, base3 + k * ncol, t);
+    gst(a.f.dq, base3 + k * ncol, q);
     if (budget) {
       const double dp = (double)a.v.dhs[k] * dpg;
       colsum[0] += (double)CP * (double)t * dp;
@@ -199,7 +214,7 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
   if (DIAG) {
     float* dg = a.f.diag + (size_t)m * diag_rows(KX) * ncol + c;
     int r = 0;
-    auto W = [&](float x) { dg[(size_t)(r++) * ncol] = x; };
+    auto W = [&](float x) { gst(dg, (size_t)(r++) * ncol, x); };
     W((float)pa.iptop); W(pa.cbmf); W(pa.precnv); W(pa.precls);
     W((float)pa.icltop); W(pa.cloudc); W(pa.clstr); W(pa.qcloud); W(pa.gse);
     W(pb.rsds); W(pb.rsns); W(pb.ftop_sw);
@@ -302,13 +317,13 @@ __device__ uint32_t collect_chunks(const KArgs& a, uint32_t m) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
   uint32_t acc = 0;
-  for (uint32_t b = b0 + lane; b <= b1; b += 64u) acc |= a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
+  for (uint32_t b = b0 + lane; b <= b1; b += 64u) acc |= gld(a.partial, b * a.jmax + (m - divq(b * a.chunk, a.dcol)));
   return wave_or<false>(acc);
 }
 
 // Two-kernel path: member m's word of level-set chunk b (levelset_batch_kernel output).
 __device__ __forceinline__ uint32_t chunk_word(const KArgs& a, uint32_t b, uint32_t m) {
-  return a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
+  return gld(a.partial, b * a.jmax + (m - divq(b * a.chunk, a.dcol)));
 }
 
 // Two-kernel path, member wider than a block: this thread's share of the OR over
@@ -353,11 +368,11 @@ __device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint
       mcol[u] = m;
 #pragma unroll
       for (int k = 1; k < KX; ++k) {
-        col[u].T[k] = a.f.t[base3 + k * ncol];
-        col[u].Q[k] = a.f.q[base3 + k * ncol];
-        col[u].PHI[k] = a.f.phi[base3 + k * ncol];
+        col[u].T[k] = gld(a.f.t, base3 + k * ncol);
+        col[u].Q[k] = gld(a.f.q, base3 + k * ncol);
+        col[u].PHI[k] = gld(a.f.phi, base3 + k * ncol);
       }
-      col[u].ps = a.f.ps[(size_t)m * ncol + c];
+      col[u].ps = gld(a.f.ps, (size_t)m * ncol + c);
     }
 #pragma unroll
     for (uint32_t u = 0; u < kLsCols; ++u) {
@@ -370,7 +385,7 @@ __device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nm; i += kBlock) a.partial[bid * a.jmax + i] = s_or[i];
+  for (uint32_t i = threadIdx.x; i < nm; i += kBlock) gst(a.partial, bid * a.jmax + i, s_or[i]);
 }
 
 // ---------------------------------------------------------------- VJP helpers
@@ -393,12 +408,12 @@ template <int KX>
 __device__ __forceinline__ void load_cotangents(const KArgs& a, uint32_t m, uint32_t c, ColumnCt<float, KX>& ct) {
   const size_t ncol = a.ncol;
   const size_t base3 = (size_t)m * KX * ncol + c;
-  ct.du1 = a.adj.ct_du[base3 + (KX - 1) * ncol];
-  ct.dv1 = a.adj.ct_dv[base3 + (KX - 1) * ncol];
+  ct.du1 = gld(a.adj.ct_du, base3 + (KX - 1) * ncol);
+  ct.dv1 = gld(a.adj.ct_dv, base3 + (KX - 1) * ncol);
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
-    ct.dt[k] = a.adj.ct_dt[base3 + k * ncol];
-    ct.dq[k] = a.adj.ct_dq[base3 + k * ncol];
+    ct.dt[k] = gld(a.adj.ct_dt, base3 + k * ncol);
+    ct.dq[k] = gld(a.adj.ct_dq, base3 + k * ncol);
   }
 }
 
@@ -408,12 +423,12 @@ __device__ __forceinline__ void store_grads(const KArgs& a, uint32_t m, uint32_t
   const size_t base3 = (size_t)m * KX * ncol + c;
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
-    a.adj.g_u[base3 + k * ncol] = k == KX - 1 ? g.u1 : 0.0f;
-    a.adj.g_v[base3 + k * ncol] = k == KX - 1 ? g.v1 : 0.0f;
-    a.adj.g_t[base3 + k * ncol] = g.T[k];
-    a.adj.g_q[base3 + k * ncol] = g.Q[k];
+    gst(a.adj.g_u, base3 + k * ncol, k == KX - 1 ? g.u1 : 0.0f);
+    gst(a.adj.g_v, base3 + k * ncol, k == KX - 1 ? g.v1 : 0.0f);
+    gst(a.adj.g_t, base3 + k * ncol, g.T[k]);
+    gst(a.adj.g_q, base3 + k * ncol, g.Q[k]);
   }
-  a.adj.g_ps[(size_t)m * ncol + c] = g.ps;
+  gst(a.adj.g_ps, (size_t)m * ncol + c, g.ps);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -450,7 +465,7 @@ __device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t bi
   if (a.ncol < (uint32_t)kBlock) {
     if (active)
 #pragma unroll
-      for (int i = 0; i < VJP_NPARAM; ++i) a.prm_part[(size_t)gid * VJP_NPARAM + i] = prm[i];
+      for (int i = 0; i < VJP_NPARAM; ++i) gst(a.prm_part, (size_t)gid * VJP_NPARAM + i, prm[i]);
     return;
   }
   __shared__ float s_red[kBlock / 64][2 * VJP_NPARAM];
@@ -467,7 +482,7 @@ __device__ __forceinline__ void block_param_partials(const KArgs& a, uint32_t bi
     float acc = s_red[0][threadIdx.x];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; ++w) acc = acc + s_red[w][threadIdx.x];
-    a.prm_part[(size_t)bid * 2 * VJP_NPARAM + threadIdx.x] = acc;
+    gst(a.prm_part, (size_t)bid * 2 * VJP_NPARAM + threadIdx.x, acc);
   }
 }
 
@@ -491,7 +506,7 @@ __device__ __forceinline__ void block_budget_partials(const KArgs& a, uint32_t b
   if (a.ncol < (uint32_t)kBlock) {
     if (active)
 #pragma unroll
-      for (int i = 0; i < kBudParts; ++i) a.bud_part[(size_t)gid * kBudParts + i] = v[i];
+      for (int i = 0; i < kBudParts; ++i) gst(a.bud_part, (size_t)gid * kBudParts + i, v[i]);
     return;
   }
   __shared__ double s_bud[kBlock / 64][2 * kBudParts];
@@ -510,7 +525,7 @@ __device__ __forceinline__ void block_budget_partials(const KArgs& a, uint32_t b
     double acc = s_bud[0][threadIdx.x];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; ++w) acc = is_max ? fmax(acc, s_bud[w][threadIdx.x]) : acc + s_bud[w][threadIdx.x];
-    a.bud_part[(size_t)bid * 2 * kBudParts + threadIdx.x] = acc;
+    gst(a.bud_part, (size_t)bid * 2 * kBudParts + threadIdx.x, acc);
   }
 }
 
@@ -591,7 +606,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
         // waits for every block's word, i.e. every block has read the generation
         for (uint32_t mm = 0; mm < a.nmember; ++mm) {
           const uint32_t v = FUSED ? collect_fused(a, tag, mm) : collect_chunks(a, mm);
-          if (threadIdx.x == 0) a.last_s[mm] = v;
+          if (threadIdx.x == 0) gst(a.last_s, mm, v);
         }
         if (FUSED && threadIdx.x == 0) {
           const uint32_t g = __hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -605,7 +620,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
     } else {   // two-kernel path, members narrower than a block: each spans <= 2 chunks
       S = 0u;
       const uint32_t b0 = divq(m * a.ncol, a.dchunk), b1 = divq(m * a.ncol + a.ncol - 1u, a.dchunk);
-      for (uint32_t b = b0; b <= b1; ++b) S |= a.partial[b * a.jmax + (m - divq(b * a.chunk, a.dcol))];
+      for (uint32_t b = b0; b <= b1; ++b) S |= gld(a.partial, b * a.jmax + (m - divq(b * a.chunk, a.dcol)));
     }
   } else if (FUSED && need_s) {
     if (threadIdx.x < 64u) {   // wave 0 gathers S for the block's member(s)
@@ -618,7 +633,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
         // block's word, i.e. every block has read the generation
         for (uint32_t mm = 0; mm < a.nmember; ++mm) {
           const uint32_t v = collect_fused_t<true>(a, tag, mm);
-          if (threadIdx.x == 0) a.last_s[mm] = v;
+          if (threadIdx.x == 0) gst(a.last_s, mm, v);
         }
         if (threadIdx.x == 0) {
           const uint32_t g = __hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -641,7 +656,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
       S = lw0;
     }
     // every member's S for speedy_read_level_sets, from its first column
-    if (active && c == 0u) a.last_s[m] = S;
+    if (active && c == 0u) gst(a.last_s, m, S);
   }
   // The forward + VJP kernels keep the zonal loads and all output stores here:
   // moving them earlier changed their register allocation and, at 16 levels,
@@ -653,7 +668,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   if (active) limited = store_outputs<KX, DIAG, EXTRA, VJP>(a, m, c, col, pa, pb, colsum);
   if (EXTRA && a.f.budget) {   // float64 column budgets (block-uniform branch)
     const uint32_t j = c - divq(c, a.dlat) * a.nlat;
-    const double w = (double)a.f.coa[j];
+    const double w = (double)gld(a.f.coa, j);
     const double E = (double)pb.evap2, P = (double)pa.precnv + (double)pa.precls;
     const double res = colsum[1] - (E - P);
     const double v[kBudParts] = {w * colsum[0], w * colsum[1], w * E, w * P, w * res, fabs(res), w};
@@ -672,7 +687,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
     if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_STATE)) {
       const size_t base3 = (size_t)m * KX * a.ncol + c;
 #pragma unroll
-      for (int k = 0; k < KX; ++k) g.Q[k] = verify_q_grad(a.f.q[base3 + k * (size_t)a.ncol], g.Q[k]);
+      for (int k = 0; k < KX; ++k) g.Q[k] = verify_q_grad(gld(a.f.q, base3 + k * (size_t)a.ncol), g.Q[k]);
     }
 #pragma unroll
     for (int k = 0; k < KX; ++k) g.Q[k] += g_qraw[k];
@@ -696,7 +711,7 @@ __device__ __forceinline__ void budget_reduce_body(const KArgs& a, uint32_t m) {
       const uint32_t j = m - divq(b * kBlock, a.dcol);
 #pragma unroll
       for (int i = 0; i < kBudParts; ++i) {
-        const double x = a.bud_part[((size_t)b * 2 + j) * kBudParts + i];
+        const double x = gld(a.bud_part, ((size_t)b * 2 + j) * kBudParts + i);
         acc[i] = i == 5 ? fmax(acc[i], x) : acc[i] + x;
       }
     }
@@ -705,7 +720,7 @@ __device__ __forceinline__ void budget_reduce_body(const KArgs& a, uint32_t m) {
     for (uint32_t cc = threadIdx.x; cc < a.ncol; cc += kBlock)
 #pragma unroll
       for (int i = 0; i < kBudParts; ++i) {
-        const double x = a.bud_part[((size_t)m * a.ncol + cc) * kBudParts + i];
+        const double x = gld(a.bud_part, ((size_t)m * a.ncol + cc) * kBudParts + i);
         acc[i] = i == 5 ? fmax(acc[i], x) : acc[i] + x;
       }
   }
@@ -728,7 +743,7 @@ __device__ __forceinline__ void budget_reduce_body(const KArgs& a, uint32_t m) {
   if (threadIdx.x < SPEEDY_NBUDGET) {
     const double wsum = s_red[0][SPEEDY_NBUDGET];
     const double t = s_red[0][threadIdx.x];
-    a.f.budget[(size_t)m * SPEEDY_NBUDGET + threadIdx.x] = threadIdx.x == 5 ? t : t / wsum;
+    gst(a.f.budget, (size_t)m * SPEEDY_NBUDGET + threadIdx.x, threadIdx.x == 5 ? t : t / wsum);
   }
 }
 
@@ -743,13 +758,13 @@ __device__ __forceinline__ void param_reduce_body(const KArgs& a, uint32_t m) {
     for (uint32_t b = b0 + threadIdx.x; b <= b1; b += kBlock) {
       const uint32_t j = m - divq(b * kBlock, a.dcol);
 #pragma unroll
-      for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += a.prm_part[((size_t)b * 2 + j) * VJP_NPARAM + i];
+      for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += gld(a.prm_part, ((size_t)b * 2 + j) * VJP_NPARAM + i);
     }
   } else {
 #pragma unroll 4
     for (uint32_t cc = threadIdx.x; cc < a.ncol; cc += kBlock)
 #pragma unroll
-      for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += a.prm_part[((size_t)m * a.ncol + cc) * VJP_NPARAM + i];
+      for (int i = 0; i < VJP_NPARAM; ++i) acc[i] += gld(a.prm_part, ((size_t)m * a.ncol + cc) * VJP_NPARAM + i);
   }
   __shared__ float s_red[kBlock / 64][VJP_NPARAM];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -763,7 +778,7 @@ __device__ __forceinline__ void param_reduce_body(const KArgs& a, uint32_t m) {
     float t = s_red[0][threadIdx.x];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; ++w) t = t + s_red[w][threadIdx.x];
-    a.adj.g_params[(size_t)m * VJP_NPARAM + threadIdx.x] = t;
+    gst(a.adj.g_params, (size_t)m * VJP_NPARAM + threadIdx.x, t);
   }
 }
 
@@ -829,8 +844,12 @@ __global__ __launch_bounds__(kBlock) void fwd_diag_batch_kernel(const KArgs* __r
   fwd_body<KX, true, false, false, true>(segs[e >> kSegShift], e & kSegLocalMask);
 }
 
+// Forward + VJP segments of a batch.  Built for two waves per SIMD at 7 and 8
+// levels (at most 256 registers a lane, 40-152 B of scratch a lane, 37 KB of LDS
+// a block): 521 -> 448 us at T425 x 8 levels, against one wave per SIMD without
+// scratch.  The fused forward + VJP kernel keeps one wave per SIMD and no scratch.
 template <int KX>
-__global__ __launch_bounds__(kBlock) void fwd_batch_kernel(const KArgs* segs, const uint32_t* table) {
+__global__ __launch_bounds__(kBlock, KX <= 8 ? 2 : 1) void fwd_batch_kernel(const KArgs* segs, const uint32_t* table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);
   fwd_body<KX, false, false, true, true>(segs[e >> kSegShift], e & kSegLocalMask);
 }

@@ -203,6 +203,31 @@ def test_fwd_diagnostics(phys):
     print(f"diagnostics: {len(checks)} fields, worst rel err {worst:.2e}", flush=True)
 
 
+@pytest.mark.parametrize("seed", [2, 9])
+def test_fwd_divisions_are_correctly_rounded(phys, seed):
+    """The column code divides with the IEEE sequence minus its range-scaling
+    steps (mdiv, speedy_column.hpp), which must equal a correctly rounded float32
+    division for every operand the physics produces.  Checked bit for bit on a
+    whole T85 grid against NumPy's float32 division on the host: the relative
+    humidity rh = Q / qsat at every level and the cloud scheme's
+    gse = (se[-2] - se[-1]) / (phi[-2] - phi[-1]), from the kernel's own qsat and
+    se diagnostics."""
+    import torch
+    case = wet(synthetic.make_case(256, 128, 8, seed=seed, land=True), seed)
+    phys.parameters = Parameters.default()
+    _, data = phys.compute_tendencies(*case)
+    torch.cuda.synchronize()
+    d = data.to_dict()
+    state = case[0]
+    qsat, rh = _np(d["humidity.qsat"]), _np(d["humidity.rh"])
+    assert qsat.dtype == np.float32 and qsat.shape == state.specific_humidity.shape
+    np.testing.assert_array_equal(rh, state.specific_humidity / qsat)
+    se, phi = _np(d["convection.se"]), state.geopotential
+    gse = (se[-2] - se[-1]) / (phi[-2] - phi[-1])
+    assert gse.dtype == np.float32
+    np.testing.assert_array_equal(_np(d["shortwave_rad.gse"]), gse)
+
+
 def test_fwd_increase_co2_snow_and_sea_ice(phys):
     """set_forcing (speedy/forcing.py:11-39) off its defaults: the CO2 trend on
     (increase_co2, 80 years after co2_year_ref), snow cover above 1 in part of the

@@ -1,17 +1,21 @@
 """Register and scratch use of the column kernels, from the compiler's gfx950
 code-object metadata (the build's own flags).
 
-The forward + VJP kernels at 7 and 8 levels run without scratch: every value
-stays in registers (VGPRs and accumulation registers).  Before, they spilled
-0.3-0.8 KB per lane, which was 2.9-3.5x the algorithmic HBM traffic of the
-call, and builds of those spilling kernels gave wrong results on the GPU after
-unrelated code changes.  The forward kernels at 7 and 8 levels fit in
-registers too, except the batched forward kernels at 8 levels: they are
-built for two waves per SIMD (at most 256 registers a lane), which costs them
-a few dwords of scratch a lane and makes the T425 forward 1.45x faster than
-one wave per SIMD without scratch.  Their scratch is bounded here instead.  A
-change that brings scratch back fails here, on the CPU, before any GPU run.
-(Sixteen-level kernels still spill; they are not checked.)
+The fused forward + VJP kernels at 7 and 8 levels run without scratch: every
+value stays in registers (VGPRs and accumulation registers).  Before, they
+spilled 0.3-0.8 KB per lane, which was 2.9-3.5x the algorithmic HBM traffic of
+the call, and builds of those spilling kernels gave wrong results on the GPU
+after unrelated code changes.  The fused and diagnostics forward kernels at 7
+and 8 levels fit in registers too.
+
+The batched kernels at 7 and 8 levels (forward-only and forward + VJP) are
+built for two waves per SIMD instead, at most 256 registers a lane.  That costs
+them some scratch (a few dwords a lane for the forward, 152 B for the 8-level
+forward + VJP) and makes them 1.45x (forward) and 1.16x (forward + VJP) faster
+at T425 than one wave per SIMD without scratch.  Their register count and
+scratch are bounded here.  A change that brings scratch back or raises it fails
+here, on the CPU, before any GPU run.  (Sixteen-level kernels still spill; they
+are not checked.)
 """
 import os
 import re
@@ -28,14 +32,14 @@ SRC = os.path.join(ROOT, "jax-gcm_amd", "csrc", "speedy_kernels.hip")
 # mangled-name prefixes of the kernels that must not use scratch
 NO_SCRATCH = {
     "fused forward + VJP, 7 and 8 levels": re.compile(r"^_ZN6speedy10fwd_kernelILi[78]ELb0ELb1ELb1ELb1E"),
-    "batched forward + VJP, 7 and 8 levels": re.compile(r"^_ZN6speedy16fwd_batch_kernelILi[78]E"),
     "forward, 7 and 8 levels": re.compile(r"^_ZN6speedy(10fwd_kernelILi[78]ELb\dELb1ELb0E|21fwd_diag_batch_kernelILi[78]E|"
                                           r"\d+fwd_(only|plain)_batch_kernelILi7E)"),
 }
 
 # kernels built for two waves per SIMD: {pattern: largest scratch bytes per lane allowed}
 TWO_WAVES = {
-    "batched forward, 7 and 8 levels": (re.compile(r"^_ZN6speedy\d+fwd_(only|plain)_batch_kernelILi[78]E"), 64),
+    "batched forward, 7 and 8 levels": (re.compile(r"^_ZN6speedy\d+fwd_(only|plain)_batch_kernelILi[78]E"), 64, 4),
+    "batched forward + VJP, 7 and 8 levels": (re.compile(r"^_ZN6speedy16fwd_batch_kernelILi[78]E"), 160, 2),
 }
 
 
@@ -46,8 +50,8 @@ def _hipcc():
     return None
 
 
-def kernel_resources():
-    """{mangled kernel name: (next free VGPR incl. accumulation registers, scratch bytes per lane)}."""
+def _device_asm(src):
+    """gfx950 assembly of one HIP source, compiled with the build's flags."""
     hipcc = _hipcc()
     if hipcc is None:
         pytest.skip("hipcc not found")
@@ -57,8 +61,12 @@ def kernel_resources():
     with tempfile.TemporaryDirectory() as d:
         asm = os.path.join(d, "k.s")
         subprocess.run([hipcc] + flags + ["--cuda-device-only", "-S", "-I", os.path.join(ROOT, "include"), "-o", asm,
-                                          SRC], check=True, capture_output=True)
-        text = open(asm).read()
+                                          src], check=True, capture_output=True)
+        return open(asm).read()
+
+
+def kernel_resources(text):
+    """{mangled kernel name: (next free VGPR incl. accumulation registers, scratch bytes per lane)}."""
     out = {}
     for blk in re.finditer(r"\.amdhsa_kernel (\S+)\n(.*?)\.end_amdhsa_kernel", text, re.S):
         body = blk.group(2)
@@ -68,8 +76,13 @@ def kernel_resources():
 
 
 @pytest.fixture(scope="module")
-def resources():
-    return kernel_resources()
+def kernels_asm():
+    return _device_asm(SRC)
+
+
+@pytest.fixture(scope="module")
+def resources(kernels_asm):
+    return kernel_resources(kernels_asm)
 
 
 @pytest.mark.parametrize("what", list(NO_SCRATCH))
@@ -83,9 +96,42 @@ def test_kernels_without_scratch(resources, what):
 
 @pytest.mark.parametrize("what", list(TWO_WAVES))
 def test_two_wave_kernels(resources, what):
-    pat, max_scratch = TWO_WAVES[what]
+    pat, max_scratch, count = TWO_WAVES[what]
     hits = {k: v for k, v in resources.items() if pat.match(k)}
-    assert len(hits) == 4, f"{what}: {sorted(hits)}"
+    assert len(hits) == count, f"{what}: {sorted(hits)}"
     for k, (vgpr, scratch) in hits.items():
         assert vgpr <= 256, f"{k}: {vgpr} registers a lane, more than two waves per SIMD allow"
         assert scratch <= max_scratch, f"{k}: scratch {scratch} bytes per lane"
+
+
+def _valu_counts(asm_text):
+    """{mangled kernel name: static count of VALU instructions} of an assembly file."""
+    out = {}
+    for m in re.finditer(r"^(_Z\w+):.*\n", asm_text, re.M):
+        end = asm_text.find(".Lfunc_end", m.end())
+        if end < 0:
+            continue
+        ops = [ln.split()[0] for ln in (x.strip() for x in asm_text[m.end():end].split("\n"))
+               if ln and not ln.startswith((".", ";")) and not ln.endswith(":")]
+        out[m.group(1)] = sum(1 for o in ops if o.startswith("v_"))
+    return out
+
+
+def test_term_census(kernels_asm):
+    """Static VALU instructions per term of the 8-level column code (gfx950,
+    the build's flags), from tests/host_column/term_census.hip: each term's kernel
+    minus the loads and stores alone.  Printed as a table; the terms together
+    must account for most of the fused forward kernel's instructions (the rest
+    is addressing, the level-set exchange and the output sums)."""
+    census = _valu_counts(_device_asm(os.path.join(ROOT, "tests", "host_column", "term_census.hip")))
+    kernels = _valu_counts(kernels_asm)
+    terms = {re.sub(r"^_Z\d+k_(\w+?)4Args.*", r"\1", k): v for k, v in census.items()}
+    base = terms.pop("copy")
+    terms = {k: v - base for k, v in terms.items()}
+    assert len(terms) == 12 and all(v >= 0 for v in terms.values()), terms
+    fused = kernels["_ZN6speedy10fwd_kernelILi8ELb0ELb1ELb0ELb0EEEvNS_5KArgsE"]
+    total = sum(terms.values())
+    for k, v in sorted(terms.items(), key=lambda kv: -kv[1]):
+        print(f"{k:18s} {v:5d}  {100.0 * v / fused:5.1f} %")
+    print(f"{'terms':18s} {total:5d}  {100.0 * total / fused:5.1f} % of fwd_kernel<8> ({fused} VALU)")
+    assert 0.75 * fused <= total <= fused
