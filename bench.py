@@ -358,18 +358,32 @@ def large_leg(phys, dev, reps=5):
 MFMA_FP32_PEAK_TFS = 157.3
 
 
+def _fourier_factors(nlon):
+    """nlon = n1 * n2 with n1 the largest divisor <= sqrt(nlon), as the library splits it."""
+    n1 = max(d for d in range(1, int(nlon ** 0.5) + 1) if nlon % d == 0)
+    return n1, nlon // n1
+
+
 def transform_work(g, kx):
     """(flops, bytes) of one transform step: dynamics -> physics state plus physics
-    tendency -> dynamics at the grid g.  Flops: 2 per multiply-add of the Fourier
-    contraction and of the Legendre contraction as computed, over half the latitudes
-    by the equatorial symmetry (5*kx+1 fields synthesised: winds, temperature,
-    humidity, log surface pressure, geopotential; 4*kx analysed).  Bytes: modal
-    inputs and outputs, nodal inputs and outputs, and each basis read once."""
-    mm, nl, nlon, nlat = 2 * g.n_m - 1, g.n_l, g.nlon, g.nlat
-    contraction = lambda nf: 2.0 * nf * mm * nlat * (nl / 2.0 + nlon)
-    flops = contraction(5 * kx + 1) + contraction(4 * kx)
+    tendency -> dynamics at the grid g (5*kx+1 fields synthesised: winds,
+    temperature, humidity, log surface pressure, geopotential; 4*kx analysed).
+    Flops, 2 per multiply-add, as the kernels compute them: the Legendre
+    contraction over half the latitudes (equatorial symmetry), and the Fourier
+    stages' two passes per latitude line (nlon = n1 * n2; analysis: n2-point real
+    DFTs, twiddles, n1-point DFTs for the n_m wavenumbers kept; synthesis the
+    reverse).  Bytes: modal inputs and outputs, nodal inputs and outputs, and the
+    Legendre basis and the roots of unity read once."""
+    mm, nl, nlon, nlat, n_m = 2 * g.n_m - 1, g.n_l, g.nlon, g.nlat, g.n_m
+    n1, n2 = _fourier_factors(nlon)
+    nh, k1 = n2 // 2 + 1, -(-n_m // n2)
+    legendre = lambda nf: 2.0 * nf * mm * nlat * (nl / 2.0)
+    analysis_line = 4.0 * n1 * nh * n2 + 6.0 * n1 * n2 + 8.0 * n_m * n1
+    synthesis_line = n2 * n1 * (8.0 * k1 + 6.0) + 4.0 * n1 * n2 * n2
+    flops = (legendre(5 * kx + 1) + (5 * kx + 1) * nlat * synthesis_line
+             + legendre(4 * kx) + 4 * kx * nlat * analysis_line)
     modal, nodal = mm * nl * 4, nlon * nlat * 4
-    bases = g.n_m * nlat * nl * 4 + nlon * mm * 4
+    bases = g.n_m * nlat * nl * 4 + nlon * 8
     nbytes = ((4 * kx + 2) * modal + (5 * kx + 1) * nodal      # dynamics -> physics
               + 4 * kx * nodal + (4 * kx + 1) * modal           # tendency -> dynamics
               + 2 * bases)

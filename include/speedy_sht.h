@@ -9,8 +9,8 @@
  * and the spherical-harmonic transforms under them (dinosaur's
  * Grid.to_modal / Grid.to_nodal with RealSphericalHarmonics, selected at
  * jcm/utils.py:28-54).  Both transform directions are a longitude Fourier
- * contraction and a latitude Legendre contraction, each one batched fp32
- * MFMA GEMM on gfx950.
+ * contraction (a truncated real DFT in two short passes) and a latitude
+ * Legendre contraction (a batched fp32 MFMA GEMM) on gfx950.
  *
  * Conventions (dinosaur RealSphericalHarmonics on a Gaussian grid):
  *   nodal field  float[nlon][nlat], latitude fastest, Gauss-Legendre latitudes
@@ -58,7 +58,9 @@ int speedy_sht_host_basis(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, 
  * wavenumbers, n_l total wavenumbers, nlon x nlat nodes, sphere radius in m).
  * Creation allocates and uploads them on the current device (synchronous);
  * it requires n_m <= n_l, 2 <= n_l <= nlat, 2*n_m-1 <= nlon and an even nlat
- * (the Legendre stages use the equatorial symmetry of the Gaussian grid). */
+ * (the Legendre stages use the equatorial symmetry of the Gaussian grid), and
+ * an nlon whose Fourier stages fit 64 KiB of LDS per workgroup (up to about
+ * 3500 longitudes; every Grid.T<N> up to T425 does). */
 int speedy_sht_create(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, double radius,
                       SpeedyShtPlan **plan);
 int speedy_sht_destroy(SpeedyShtPlan *plan);

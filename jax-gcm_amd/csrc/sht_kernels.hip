@@ -9,17 +9,20 @@
 // Data flow of one conversion (B stacked fields, Mm = 2*n_m-1 modes):
 //   to_nodal:  modal [B][Mm][L] --Legendre--> Y [B][Mm][nlat] --Fourier--> nodal [B][nlon][nlat]
 //   to_modal:  nodal [B][nlon][nlat] --Fourier--> Y [B][Mm][nlat] --Legendre--> modal [B][Mm][L]
-// Each arrow is one launch of sht_gemm_kernel<STAGE>: a 64x64 output tile per
-// workgroup of four waves, each wave 32x32 as 2x2 v_mfma_f32_16x16x4f32, K in
-// steps of 32 staged through two LDS stages.  The Fourier stages batch over fields (the
-// basis F is shared), the Legendre stages batch over modes (the basis P[m] is
-// shared by every field).  Per-field pre/post operations (1/cos(lat), unit
+// Each Legendre arrow is one launch of sht_gemm_kernel<STAGE>: a 64x64 output
+// tile per workgroup of four waves, each wave 32x32 as 2x2
+// v_mfma_f32_16x16x4f32, K in steps of 32 staged through two LDS stages,
+// batched over modes (the basis P[m] is shared by every field).  Each Fourier
+// arrow is one launch of fourier_analysis_kernel / fourier_synthesis_kernel: a
+// real DFT over longitude factored into two short passes in LDS, batched over
+// fields and latitude tiles.  Per-field pre/post operations (1/cos(lat), unit
 // scales, reference temperature, log/exp of surface pressure) ride in the
-// Fourier stages' tile loads and stores; the l +- 1 modal operators are one
+// Fourier stages' loads and stores; the l +- 1 modal operators are one
 // elementwise kernel on either side.
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -32,10 +35,12 @@ struct SpeedyShtPlan {
   int32_t n_m, n_l, nlon, nlat, mm;   // mm = 2*n_m-1 modes
   double radius;
   int device;
-  float* F;     // [nlon][mm]
+  int32_t n1, n2;      // nlon = n1 * n2 (the Fourier stages' two passes)
+  int32_t tl_a, tl_s;  // latitudes per workgroup of the Fourier analysis / synthesis
   float* P;     // [n_m][nlat][n_l]
   float* w;     // [nlat] Gauss weights * 2 pi / nlon
   float* sec;   // [nlat] 1 / cos(lat)
+  float* tw;    // [2][nlon]: cos(2 pi j / nlon), -sin(2 pi j / nlon)
 };
 
 namespace sht {
@@ -50,7 +55,7 @@ int fail(int code, const std::string& msg) {
 constexpr int kThreads = 256, kBM = 64, kBN = 64, kBK = 32, kPad = 4;
 constexpr int kMaxSeg = 8;
 
-enum Stage { kFourierAnalysis = 0, kLegendreAnalysis = 1, kLegendreSynthesis = 2, kFourierSynthesis = 3 };
+enum Stage { kLegendreAnalysis = 1, kLegendreSynthesis = 2 };
 enum : int32_t { kOpLog = 1, kOpExp = 2, kOpSecLat = 4, kOpSubLevel = 8, kOpAddLevel = 16 };
 
 // A run of stacked fields that share a source (Fourier analysis) or a
@@ -69,13 +74,9 @@ struct SegTable {
 
 struct StageArgs {
   int32_t nlon, nlat, mm, nl, nfield;
-  const float* F;
   const float* P;
-  const float* w;
-  const float* sec;
-  const float* in;    // LA: Y, LS: modal X, FS: Y
-  float* out;         // FA: Y, LA: modal, LS: Y
-  SegTable seg;       // FA: sources, FS: destinations (sorted by first)
+  const float* in;    // LA: Y, LS: modal X
+  float* out;         // LA: modal, LS: Y halves
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -91,58 +92,38 @@ __device__ __forceinline__ int find_seg(const SegTable& t, int f) {
 // (r, c) of X at X[r * x_r + c * x_c].
 template <int STAGE>
 __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
-  // Legendre stages: blockIdx.y = 2 * mode + parity.  Gaussian latitudes are
+  // blockIdx.y = 2 * mode + parity.  Gaussian latitudes are
   // symmetric (x_{nlat-1-j} = -x_j) and P_l^m(-x) = (-1)^(l+m) P_l^m(x), so each
   // mode splits into the l with l+m even (parity 0) and odd (parity 1), each over
   // the southern half of the latitudes: half the work and half the K of a full
   // contraction.  Analysis reads Y_j +- Y_{nlat-1-j}; synthesis writes the even
   // and odd partial sums E, O for the southern half ([parity][field][mode][h]),
   // which the Fourier synthesis combines as E + O (south) and E - O (north).
-  const bool leg = STAGE == kLegendreAnalysis || STAGE == kLegendreSynthesis;
-  const int b = leg ? (int)(blockIdx.y >> 1) : (int)blockIdx.y;
-  const int par = leg ? (int)(blockIdx.y & 1u) : 0;
+  const int b = (int)(blockIdx.y >> 1);
+  const int par = (int)(blockIdx.y & 1u);
   const int h = a.nlat >> 1;
   int M, N, K;
   const float* A;
   const float* B;
   float* C;
   long a_m, a_k, b_k, b_n, c_m, c_n = 1;
-  const int mode_m = (b + 1) >> 1;    // Legendre stages: order m of mode b
+  const int mode_m = (b + 1) >> 1;    // order m of mode b
   const int lofs = (mode_m + par) & 1;   // first l of this parity (l = lofs + 2c)
   const int npar = (a.nl - lofs + 1) >> 1;
-  Seg sg{};
-  float lvl = 0.0f;
-  if (STAGE == kFourierAnalysis) {       // Y_b[mm x nlat] = F^T[mm x nlon] * x_b[nlon x nlat]
-    sg = a.seg.s[find_seg(a.seg, b)];
-    M = a.mm; N = a.nlat; K = a.nlon;
-    A = a.F; a_m = 1; a_k = a.mm;
-    B = sg.src + (size_t)(b - sg.first) * a.nlon * a.nlat; b_k = a.nlat; b_n = 1;
-    C = a.out + (size_t)b * a.mm * a.nlat; c_m = a.nlat;
-    if (sg.level) lvl = sg.level[(b - sg.first) % sg.period];
-  } else if (STAGE == kLegendreAnalysis) {
+  if (STAGE == kLegendreAnalysis) {
     // Z[:, b, lofs + 2c] [nfield x npar] = (Y_j +- Y_{nlat-1-j})[:, b, :h] [nfield x h] * P_m[:h, lofs + 2c]
     M = a.nfield; N = npar; K = h;
     A = a.in + (size_t)b * a.nlat; a_m = (long)a.mm * a.nlat; a_k = 1;
     B = a.P + (size_t)mode_m * a.nlat * a.nl + lofs; b_k = a.nl; b_n = 2;
     C = a.out + (size_t)b * a.nl + lofs; c_m = (long)a.mm * a.nl; c_n = 2;
-  } else if (STAGE == kLegendreSynthesis) {
+  } else {
     // E|O[par][:, b, :h] [nfield x h] = X[:, b, lofs + 2c] [nfield x npar] * P_m[:h, lofs + 2c]^T
     M = a.nfield; N = h; K = npar;
     A = a.in + (size_t)b * a.nl + lofs; a_m = (long)a.mm * a.nl; a_k = 2;
     B = a.P + (size_t)mode_m * a.nlat * a.nl + lofs; b_k = 2; b_n = a.nl;
     C = a.out + (size_t)par * a.nfield * a.mm * h + (size_t)b * h; c_m = (long)a.mm * h;
-  } else {                                   // out_b [nlon x nlat] = F [nlon x mm] * Y_b [mm x nlat]
-    sg = a.seg.s[find_seg(a.seg, b)];
-    M = a.nlon; N = a.nlat; K = a.mm;
-    A = a.F; a_m = a.mm; a_k = 1;
-    // Y_b from the even / odd halves: [par][field][mode][h]
-    B = a.in + (size_t)b * a.mm * h; b_k = h; b_n = 1;
-    C = sg.dst + (size_t)(b - sg.first) * a.nlon * a.nlat; c_m = a.nlat;
-    if (sg.level) lvl = sg.level[(b - sg.first) % sg.period];
   }
   const float sgn = par ? -1.0f : 1.0f;
-  const size_t odd_off = (size_t)a.nfield * a.mm * h;   // Fourier synthesis: O after E
-  constexpr bool kAKContig = STAGE != kFourierAnalysis;
   constexpr bool kBNContig = STAGE != kLegendreSynthesis;
   constexpr int kLoads = kBM * kBK / kThreads;
 
@@ -170,21 +151,13 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  // Fourier analysis: x -> (op(x) - level) * mul * w(lat) [* sec(lat)], applied when
-  // the prefetched values go to LDS (after the MFMAs of the current step), so the
-  // loads stay in flight; the column of a thread's B elements is fixed (kBN | kThreads).
-  float colscale = 1.0f;
-  if (STAGE == kFourierAnalysis) {
-    const int gn = n0 + t % kBN;
-    colscale = gn < N ? sg.mul * a.w[gn] * ((sg.op & kOpSecLat) ? a.sec[gn] : 1.0f) : 0.0f;
-  }
   float ra[kLoads], rb[kLoads];
   auto load = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < kLoads; ++r) {
       const int e = t + kThreads * r;
-      const int mm = kAKContig ? (e / kBK) : (e % kBM);
-      const int kk = kAKContig ? (e % kBK) : (e / kBM);
+      const int mm = e / kBK;
+      const int kk = e % kBK;
       // clamped indices (always in bounds), no condition: the loads issue back to
       // back; elements outside the matrix are zeroed when they go to LDS
       const long row = min(m0 + mm, M - 1) * a_m;
@@ -200,31 +173,19 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
       const int nn = kBNContig ? (e % kBN) : (e / kBK);
       const int kk = kBNContig ? (e / kBN) : (e % kBK);
       const int ck = min(k0 + kk, K - 1), cn = min(n0 + nn, N - 1);
-      if (STAGE == kFourierSynthesis) {   // latitude cn from E + O (south) or E - O (north)
-        const int j = cn < h ? cn : a.nlat - 1 - cn;
-        const float* e = B + (size_t)ck * h + j;
-        rb[r] = cn < h ? e[0] + e[odd_off] : e[0] - e[odd_off];
-      } else {
-        rb[r] = B[ck * b_k + cn * b_n];
-      }
+      rb[r] = B[ck * b_k + cn * b_n];
     }
   };
   auto store = [&](int st, int k0) {
 #pragma unroll
     for (int r = 0; r < kLoads; ++r) {
       const int e = t + kThreads * r;
-      const int am = kAKContig ? (e / kBK) : (e % kBM);
-      const int ak = kAKContig ? (e % kBK) : (e / kBM);
+      const int am = e / kBK;
+      const int ak = e % kBK;
       As[st][am][ak] = (m0 + am < M && k0 + ak < K) ? ra[r] : 0.0f;
       const int nn = kBNContig ? (e % kBN) : (e / kBK);
       const int kk = kBNContig ? (e / kBN) : (e % kBK);
-      float v = rb[r];
-      if (STAGE == kFourierAnalysis) {
-        if (sg.op & kOpLog) v = logf(v);
-        if (sg.op & kOpSubLevel) v -= lvl;
-        v *= colscale;
-      }
-      Bs[st][nn][kk] = (k0 + kk < K && n0 + nn < N) ? v : 0.0f;
+      Bs[st][nn][kk] = (k0 + kk < K && n0 + nn < N) ? rb[r] : 0.0f;
     }
   };
   load(0);
@@ -270,16 +231,401 @@ __global__ __launch_bounds__(kThreads) void sht_gemm_kernel(StageArgs a) {
       for (int r = 0; r < 4; ++r) {
         const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
         if (gm >= M) continue;
-        float v = acc[i][j][r];
-        if (STAGE == kFourierSynthesis) {
-          if (sg.op & kOpSecLat) v *= a.sec[gn];
-          v = v * sg.mul;
-          if (sg.op & kOpAddLevel) v += lvl;
-          if (sg.op & kOpExp) v = expf(v);
-        }
-        C[gm * c_m + gn * c_n] = v;
+        C[gm * c_m + gn * c_n] = acc[i][j][r];
       }
     }
+}
+
+// ------------------------------------------------------------ Fourier stages
+// The longitude contraction is a real DFT of length N = nlon truncated to the
+// n_m lowest wavenumbers, computed in two short passes (Cooley-Tukey with
+// N = n1 * n2, longitude i = i1 + n1 * i2, wavenumber k = k2 + n2 * k1):
+//   X_k = sum_i1 w^(i1 k) Z[i1][k mod n2],   Z[i1][k2] = sum_i2 x[i1 + n1 i2] w_n2^(i2 k2),
+// with w = exp(-2 pi i / N) and w_n2 = w^n1.  Z of a real line is Hermitian in
+// k2, so only k2 <= n2 / 2 is formed.  The modal coefficients are the basis's
+// projections: mode 0 = X_0 / sqrt(2 pi), mode 2k-1 = Re X_k / sqrt(pi), mode
+// 2k = -Im X_k / sqrt(pi).  Synthesis runs the same passes backwards.  About
+// 4x fewer flops than the dense contraction with the basis, and two passes of
+// length n1 and n2 instead of one of length nlon.  A workgroup takes one field
+// and `tl` latitudes (latitude fastest, as in memory); the line tile, the
+// intermediate Z or W and the N roots of unity live in LDS.
+struct FourierArgs {
+  int32_t nlon, nlat, mm, n_m, n1, n2, tl, nfield;
+  const float* twr;     // [nlon] cos(2 pi j / nlon)
+  const float* twi;     // [nlon] -sin(2 pi j / nlon)
+  const float* w;       // [nlat] Gauss weights * 2 pi / nlon
+  const float* sec;     // [nlat]
+  const float* in;      // synthesis: E / O halves [2][nfield][mm][nlat / 2]
+  float* out;           // analysis: Y [nfield][mm][nlat]
+  SegTable seg;
+};
+
+constexpr float kRsqrt2Pi = 0.3989422804014327f;   // 1 / sqrt(2 pi)
+constexpr float kRsqrtPi = 0.5641895835477563f;    // 1 / sqrt(pi)
+
+__device__ __forceinline__ void load_roots(const FourierArgs& a, float* twr, float* twi) {
+  for (int i = threadIdx.x; i < a.nlon; i += kThreads) {
+    twr[i] = a.twr[i];
+    twi[i] = a.twi[i];
+  }
+}
+
+// Y_b[mode][lat] = sum_lon F[lon][mode] * prologue(x_b[lon][lat]) for one field b
+// (blockIdx.y) and latitudes j0 .. j0 + tl - 1 (blockIdx.x).
+__global__ __launch_bounds__(kThreads) void fourier_analysis_generic(FourierArgs a) {
+  extern __shared__ float lds[];
+  const int N = a.nlon, n1 = a.n1, n2 = a.n2, nh = n2 / 2 + 1, tl = a.tl;
+  const int b = blockIdx.y, j0 = blockIdx.x * tl;
+  float* twr = lds;
+  float* twi = twr + N;
+  float* xs = twi + N;            // [N][tl]
+  float* zr = xs + N * tl;        // [n1][nh][tl]
+  float* zi = zr + n1 * nh * tl;
+  const Seg sg = a.seg.s[find_seg(a.seg, b)];
+  const float* src = sg.src + (size_t)(b - sg.first) * N * a.nlat;
+  const float lvl = sg.level ? sg.level[(b - sg.first) % sg.period] : 0.0f;
+  load_roots(a, twr, twi);
+  for (int e = threadIdx.x; e < N * tl; e += kThreads) {
+    const int lon = e / tl, jj = e - lon * tl, j = j0 + jj;
+    float v = 0.0f;
+    if (j < a.nlat) {
+      v = src[(size_t)lon * a.nlat + j];
+      if (sg.op & kOpLog) v = logf(v);
+      if (sg.op & kOpSubLevel) v -= lvl;
+      v *= sg.mul * a.w[j] * ((sg.op & kOpSecLat) ? a.sec[j] : 1.0f);
+    }
+    xs[e] = v;
+  }
+  __syncthreads();
+  // pass 1: Z[i1][k2] = sum_i2 x[i1 + n1 i2] w^(n1 i2 k2), k2 <= n2 / 2
+  for (int e = threadIdx.x; e < n1 * nh * tl; e += kThreads) {
+    const int r = e / tl, jj = e - r * tl, k2 = r % nh, i1 = r / nh;
+    float re = 0.0f, im = 0.0f;
+    int q = 0;   // (i2 * k2) mod n2
+    for (int i2 = 0; i2 < n2; ++i2) {
+      const float x = xs[(i1 + n1 * i2) * tl + jj];
+      re += x * twr[q * n1];
+      im += x * twi[q * n1];
+      q += k2;
+      if (q >= n2) q -= n2;
+    }
+    zr[e] = re;
+    zi[e] = im;
+  }
+  __syncthreads();
+  // pass 2: X_k = sum_i1 w^(i1 k) Z[i1][k mod n2] for k < n_m, then the modes
+  float* Y = a.out + (size_t)b * a.mm * a.nlat;
+  for (int e = threadIdx.x; e < a.n_m * tl; e += kThreads) {
+    const int k = e / tl, jj = e - k * tl, j = j0 + jj;
+    int k2 = k % n2;
+    const bool cj = k2 >= nh;          // Z[i1][k2] = conj(Z[i1][n2 - k2])
+    if (cj) k2 = n2 - k2;
+    const float sgn = cj ? -1.0f : 1.0f;
+    float re = 0.0f, im = 0.0f;
+    int q = 0;   // (i1 * k) mod N
+    const int kq = k % N;
+    for (int i1 = 0; i1 < n1; ++i1) {
+      const int z = (i1 * nh + k2) * tl + jj;
+      const float a_r = zr[z], a_i = sgn * zi[z];
+      const float t_r = twr[q], t_i = twi[q];
+      re += a_r * t_r - a_i * t_i;
+      im += a_r * t_i + a_i * t_r;
+      q += kq;
+      if (q >= N) q -= N;
+    }
+    if (j < a.nlat) {
+      if (k == 0) {
+        Y[j] = re * kRsqrt2Pi;
+      } else {
+        Y[(size_t)(2 * k - 1) * a.nlat + j] = re * kRsqrtPi;
+        Y[(size_t)(2 * k) * a.nlat + j] = -im * kRsqrtPi;
+      }
+    }
+  }
+}
+
+// out_b[lon][lat] = epilogue(sum_mode F[lon][mode] * Y_b[mode][lat]), Y_b from the
+// Legendre synthesis's even / odd halves: E + O (southern latitudes), E - O
+// (northern).
+__global__ __launch_bounds__(kThreads) void fourier_synthesis_generic(FourierArgs a) {
+  extern __shared__ float lds[];
+  const int N = a.nlon, n1 = a.n1, n2 = a.n2, tl = a.tl, h = a.nlat >> 1;
+  const int b = blockIdx.y, j0 = blockIdx.x * tl;
+  float* twr = lds;
+  float* twi = twr + N;
+  float* ys = twi + N;             // [mm][tl]
+  float* wr = ys + a.mm * tl;      // [n1][n2][tl]
+  float* wi = wr + N * tl;
+  const Seg sg = a.seg.s[find_seg(a.seg, b)];
+  load_roots(a, twr, twi);
+  const float* E = a.in + (size_t)b * a.mm * h;
+  const size_t odd = (size_t)a.nfield * a.mm * h;
+  for (int e = threadIdx.x; e < a.mm * tl; e += kThreads) {
+    const int mode = e / tl, jj = e - mode * tl, j = j0 + jj;
+    float v = 0.0f;
+    if (j < a.nlat) {
+      const float* p = E + (size_t)mode * h + (j < h ? j : a.nlat - 1 - j);
+      v = j < h ? p[0] + p[odd] : p[0] - p[odd];
+    }
+    ys[e] = v;
+  }
+  __syncthreads();
+  // pass 1: W[i1][k2] = sum_{k = k2 + n2 k1 < n_m} c_k conj(w^(i1 k)),
+  // c_0 = Y_0 / sqrt(2 pi), c_k = (Y_{2k-1} - i Y_{2k}) / sqrt(pi)
+  for (int e = threadIdx.x; e < N * tl; e += kThreads) {
+    const int r = e / tl, jj = e - r * tl, k2 = r % n2, i1 = r / n2;
+    float re = 0.0f, im = 0.0f;
+    for (int k = k2; k < a.n_m; k += n2) {
+      float c_r, c_i;
+      if (k == 0) {
+        c_r = ys[jj] * kRsqrt2Pi;
+        c_i = 0.0f;
+      } else {
+        c_r = ys[(2 * k - 1) * tl + jj] * kRsqrtPi;
+        c_i = -ys[(2 * k) * tl + jj] * kRsqrtPi;
+      }
+      const int q = (int)(((long)i1 * k) % N);
+      const float t_r = twr[q], t_i = -twi[q];
+      re += c_r * t_r - c_i * t_i;
+      im += c_r * t_i + c_i * t_r;
+    }
+    wr[e] = re;
+    wi[e] = im;
+  }
+  __syncthreads();
+  // pass 2: x[i1 + n1 i2] = Re sum_k2 W[i1][k2] conj(w^(n1 i2 k2)), then the epilogue
+  float* dst = sg.dst + (size_t)(b - sg.first) * N * a.nlat;
+  const float lvl = sg.level ? sg.level[(b - sg.first) % sg.period] : 0.0f;
+  for (int e = threadIdx.x; e < N * tl; e += kThreads) {
+    // latitude fastest, then i2: the lanes of a wave share i1, so their W reads
+    // are broadcasts or distinct banks
+    const int r = e / tl, jj = e - r * tl, j = j0 + jj;
+    const int i2 = r % n2, i1 = r / n2, lon = i1 + n1 * i2;
+    float v = 0.0f;
+    int q = 0;   // (i2 * k2) mod n2
+    for (int k2 = 0; k2 < n2; ++k2) {
+      const int z = (i1 * n2 + k2) * tl + jj;
+      v += wr[z] * twr[q * n1] + wi[z] * twi[q * n1];
+      q += i2;
+      if (q >= n2) q -= n2;
+    }
+    if (j < a.nlat) {
+      if (sg.op & kOpSecLat) v *= a.sec[j];
+      v = v * sg.mul;
+      if (sg.op & kOpAddLevel) v += lvl;
+      if (sg.op & kOpExp) v = expf(v);
+      dst[(size_t)lon * a.nlat + j] = v;
+    }
+  }
+}
+
+// The same two passes with both factors known at compile time (the nlon of the
+// Grid.T<N> table): each pass is a short dense DFT in registers whose roots are
+// literals (Roots<n>, evaluated by the compiler), with the line tile read from
+// and written to memory directly; only the pass-1 results and the N roots of
+// unity for the twiddles w^(i1 k2) go through LDS.
+constexpr double kPiD = 3.14159265358979323846;
+constexpr double ct_sin(double x) {   // |x| <= pi, Taylor series to 1e-16
+  double term = x, sum = x;
+  for (int n = 1; n < 24; ++n) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ct_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 24; ++n) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+// w_n^q = exp(-2 pi i q / n) = c[q] + i s[q]
+template <int n>
+struct Roots {
+  float c[n], s[n];
+  constexpr Roots() : c(), s() {
+    for (int q = 0; q < n; ++q) {
+      double t = 2.0 * kPiD * q / n;
+      if (t > kPiD) t -= 2.0 * kPiD;
+      c[q] = (float)ct_cos(t);
+      s[q] = (float)-ct_sin(t);
+    }
+  }
+};
+
+template <int N1, int N2>
+__global__ __launch_bounds__(kThreads) void fourier_analysis_kernel(FourierArgs a) {
+  constexpr int N = N1 * N2, NH = N2 / 2 + 1;
+  constexpr Roots<N1> R1{};
+  constexpr Roots<N2> R2{};
+  extern __shared__ float lds[];
+  const int tl = a.tl, b = blockIdx.y, j0 = blockIdx.x * tl;
+  float* twr = lds;
+  float* twi = twr + N;
+  float* zr = twi + N;            // [NH][N1][tl]
+  float* zi = zr + NH * N1 * tl;
+  const Seg sg = a.seg.s[find_seg(a.seg, b)];
+  const float* src = sg.src + (size_t)(b - sg.first) * N * a.nlat;
+  const float lvl = sg.level ? sg.level[(b - sg.first) % sg.period] : 0.0f;
+  load_roots(a, twr, twi);
+  // pass 1: thread (i1, lat): Z[i1][k2] = sum_i2 x[i1 + N1 i2] w_N2^(i2 k2), k2 <= N2 / 2
+  for (int e = threadIdx.x; e < N1 * tl; e += kThreads) {
+    const int i1 = e / tl, jj = e - i1 * tl, j = j0 + jj;
+    const bool in = j < a.nlat;
+    const int jc = in ? j : 0;
+    const float cs = in ? sg.mul * a.w[jc] * ((sg.op & kOpSecLat) ? a.sec[jc] : 1.0f) : 0.0f;
+    float x[N2];
+#pragma unroll
+    for (int i2 = 0; i2 < N2; ++i2) x[i2] = src[(size_t)(i1 + N1 * i2) * a.nlat + jc];
+#pragma unroll
+    for (int i2 = 0; i2 < N2; ++i2) {
+      float v = x[i2];
+      if (sg.op & kOpLog) v = logf(v);
+      if (sg.op & kOpSubLevel) v -= lvl;
+      x[i2] = in ? v * cs : 0.0f;
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < NH; ++k2) {
+      float re = 0.0f, im = 0.0f;
+#pragma unroll
+      for (int i2 = 0; i2 < N2; ++i2) {
+        re += x[i2] * R2.c[(i2 * k2) % N2];
+        im += x[i2] * R2.s[(i2 * k2) % N2];
+      }
+      zr[(k2 * N1 + i1) * tl + jj] = re;
+      zi[(k2 * N1 + i1) * tl + jj] = im;
+    }
+  }
+  __syncthreads();
+  // pass 2: thread (k2, lat): X_{k2 + N2 k1} = sum_i1 (Z[i1][k2] w^(i1 k2)) w_N1^(i1 k1)
+  float* Y = a.out + (size_t)b * a.mm * a.nlat;
+  for (int e = threadIdx.x; e < N2 * tl; e += kThreads) {
+    const int k2 = e / tl, jj = e - k2 * tl, j = j0 + jj;
+    const bool cj = k2 >= NH;        // Z[i1][k2] = conj(Z[i1][N2 - k2])
+    const int kz = cj ? N2 - k2 : k2;
+    float pr[N1], pi[N1];
+#pragma unroll
+    for (int i1 = 0; i1 < N1; ++i1) {
+      const int z = (kz * N1 + i1) * tl + jj;
+      const float a_r = zr[z], a_i = cj ? -zi[z] : zi[z];
+      const int q = (i1 * k2) % N;
+      const float t_r = twr[q], t_i = twi[q];
+      pr[i1] = a_r * t_r - a_i * t_i;
+      pi[i1] = a_r * t_i + a_i * t_r;
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      if (N2 * k1 >= a.n_m) break;   // uniform; k < n_m is masked per lane below
+      const int k = k2 + N2 * k1;
+      float re = 0.0f, im = 0.0f;
+#pragma unroll
+      for (int i1 = 0; i1 < N1; ++i1) {
+        const float c = R1.c[(i1 * k1) % N1], s = R1.s[(i1 * k1) % N1];
+        re += pr[i1] * c - pi[i1] * s;
+        im += pr[i1] * s + pi[i1] * c;
+      }
+      if (j < a.nlat && k < a.n_m) {
+        if (k == 0) {
+          Y[j] = re * kRsqrt2Pi;
+        } else {
+          Y[(size_t)(2 * k - 1) * a.nlat + j] = re * kRsqrtPi;
+          Y[(size_t)(2 * k) * a.nlat + j] = -im * kRsqrtPi;
+        }
+      }
+    }
+  }
+}
+
+template <int N1, int N2>
+__global__ __launch_bounds__(kThreads) void fourier_synthesis_kernel(FourierArgs a) {
+  constexpr int N = N1 * N2;
+  constexpr Roots<N1> R1{};
+  constexpr Roots<N2> R2{};
+  extern __shared__ float lds[];
+  const int tl = a.tl, h = a.nlat >> 1, b = blockIdx.y, j0 = blockIdx.x * tl;
+  float* twr = lds;
+  float* twi = twr + N;
+  float* wr = twi + N;             // [N1][N2][tl]
+  float* wi = wr + N * tl;
+  const Seg sg = a.seg.s[find_seg(a.seg, b)];
+  load_roots(a, twr, twi);
+  __syncthreads();
+  const float* E = a.in + (size_t)b * a.mm * h;
+  const size_t odd = (size_t)a.nfield * a.mm * h;
+  // pass 1: thread (k2, lat): W[i1][k2] = conj(w^(i1 k2)) sum_k1 c_{k2 + N2 k1} conj(w_N1^(i1 k1)),
+  // c_0 = Y_0 / sqrt(2 pi), c_k = (Y_{2k-1} - i Y_{2k}) / sqrt(pi); Y from E + O (south), E - O (north)
+  for (int e = threadIdx.x; e < N2 * tl; e += kThreads) {
+    const int k2 = e / tl, jj = e - k2 * tl, j = j0 + jj;
+    const bool in = j < a.nlat;
+    const int jh = !in ? 0 : (j < h ? j : a.nlat - 1 - j);
+    const float so = j < h ? 1.0f : -1.0f;
+    // the wavenumbers k = k2 + N2 k1 < n_m: k1 < ceil(n_m / N2) for every lane
+    // (a uniform bound), the last one masked per lane; loads at clamped modes
+    // (in bounds) issue back to back, all before any use
+    float ec[N1], oc[N1], es[N1], os[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      if (N2 * k1 >= a.n_m) break;
+      const int k = k2 + N2 * k1;
+      const int kc = k < a.n_m ? k : 0;
+      const float* pc = E + (size_t)(kc == 0 ? 0 : 2 * kc - 1) * h + jh;
+      const float* ps = E + (size_t)(kc == 0 ? 0 : 2 * kc) * h + jh;
+      ec[k1] = pc[0]; oc[k1] = pc[odd]; es[k1] = ps[0]; os[k1] = ps[odd];
+    }
+    float cr[N1], ci[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      cr[k1] = 0.0f;
+      ci[k1] = 0.0f;
+      if (N2 * k1 >= a.n_m) continue;
+      const int k = k2 + N2 * k1;
+      const bool use = in && k < a.n_m;
+      const float y_c = ec[k1] + so * oc[k1], y_s = es[k1] + so * os[k1];
+      cr[k1] = use ? y_c * (k == 0 ? kRsqrt2Pi : kRsqrtPi) : 0.0f;
+      ci[k1] = use && k > 0 ? -y_s * kRsqrtPi : 0.0f;
+    }
+#pragma unroll
+    for (int i1 = 0; i1 < N1; ++i1) {
+      float re = 0.0f, im = 0.0f;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const float c = R1.c[(i1 * k1) % N1], s = -R1.s[(i1 * k1) % N1];
+        re += cr[k1] * c - ci[k1] * s;
+        im += cr[k1] * s + ci[k1] * c;
+      }
+      const int q = (i1 * k2) % N;
+      const float t_r = twr[q], t_i = -twi[q];
+      wr[(i1 * N2 + k2) * tl + jj] = re * t_r - im * t_i;
+      wi[(i1 * N2 + k2) * tl + jj] = re * t_i + im * t_r;
+    }
+  }
+  __syncthreads();
+  // pass 2: thread (i1, lat): x[i1 + N1 i2] = Re sum_k2 W[i1][k2] conj(w_N2^(i2 k2)), then the epilogue
+  float* dst = sg.dst + (size_t)(b - sg.first) * N * a.nlat;
+  const float lvl = sg.level ? sg.level[(b - sg.first) % sg.period] : 0.0f;
+  for (int e = threadIdx.x; e < N1 * tl; e += kThreads) {
+    const int i1 = e / tl, jj = e - i1 * tl, j = j0 + jj;
+    const int jc = j < a.nlat ? j : 0;
+    float pr[N2], pi[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      pr[k2] = wr[(i1 * N2 + k2) * tl + jj];
+      pi[k2] = wi[(i1 * N2 + k2) * tl + jj];
+    }
+#pragma unroll
+    for (int i2 = 0; i2 < N2; ++i2) {
+      float v = 0.0f;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) v += pr[k2] * R2.c[(i2 * k2) % N2] + pi[k2] * R2.s[(i2 * k2) % N2];
+      if (sg.op & kOpSecLat) v *= a.sec[jc];
+      v = v * sg.mul;
+      if (sg.op & kOpAddLevel) v += lvl;
+      if (sg.op & kOpExp) v = expf(v);
+      if (j < a.nlat) dst[(size_t)(i1 + N1 * i2) * a.nlat + j] = v;
+    }
+  }
 }
 
 // eps(m, l) = sqrt((l^2 - m^2) / (4 l^2 - 1)) for m <= l, l >= 1; else 0:
@@ -523,36 +869,81 @@ Ws carve(const SpeedyShtPlan* p, int32_t nfield, void* ws) {
 StageArgs stage_args(const SpeedyShtPlan* p, int32_t nfield) {
   StageArgs a{};
   a.nlon = p->nlon; a.nlat = p->nlat; a.mm = p->mm; a.nl = p->n_l; a.nfield = nfield;
-  a.F = p->F; a.P = p->P; a.w = p->w; a.sec = p->sec;
+  a.P = p->P;
   return a;
+}
+
+FourierArgs fourier_args(const SpeedyShtPlan* p, int32_t nfield, int32_t tl, const SegTable& seg) {
+  FourierArgs f{};
+  f.nlon = p->nlon; f.nlat = p->nlat; f.mm = p->mm; f.n_m = p->n_m; f.n1 = p->n1; f.n2 = p->n2; f.tl = tl;
+  f.nfield = nfield;
+  f.twr = p->tw; f.twi = p->tw + p->nlon; f.w = p->w; f.sec = p->sec;
+  f.seg = seg;
+  return f;
+}
+
+// The Fourier kernels of one factor pair nlon = n1 * n2: compiled for the nlon
+// of the Grid.T<N> table up to T213 (nlon 640), the generic LDS kernels
+// otherwise (at n1 = 32 the register passes would spill to scratch).
+typedef void (*FourierKernel)(FourierArgs);
+struct FourierPair {
+  int n1, n2;
+  FourierKernel analysis, synthesis;
+};
+#define SHT_PAIR(a, b) {a, b, fourier_analysis_kernel<a, b>, fourier_synthesis_kernel<a, b>}
+const FourierPair kPairs[] = {SHT_PAIR(8, 8),   SHT_PAIR(8, 12),  SHT_PAIR(8, 16),  SHT_PAIR(16, 16),
+                              SHT_PAIR(16, 20), SHT_PAIR(18, 20), SHT_PAIR(16, 32), SHT_PAIR(20, 32)};
+#undef SHT_PAIR
+
+const FourierPair* find_pair(int n1, int n2) {
+  for (const FourierPair& f : kPairs)
+    if (f.n1 == n1 && f.n2 == n2) return &f;
+  return nullptr;
+}
+
+// LDS bytes of the Fourier kernels (roots, line tile, pass-1 result)
+constexpr size_t kFourierLds = 64 * 1024;
+size_t analysis_lds(bool fixed, int N, int n1, int n2, int tl) {
+  const size_t z = 2 * (size_t)n1 * (n2 / 2 + 1) * tl;
+  return 4 * (2 * (size_t)N + z + (fixed ? 0 : (size_t)N * tl));
+}
+size_t synthesis_lds(bool fixed, int N, int mm, int tl) {
+  return 4 * (2 * (size_t)N + 2 * (size_t)N * tl + (fixed ? 0 : (size_t)mm * tl));
 }
 
 int tiles(int M, int N) { return ((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN); }
 
-hipError_t run_analysis(const SpeedyShtPlan* p, StageArgs a, float* y, float* modal_out, hipStream_t s) {
-  a.out = y;
-  hipLaunchKernelGGL(sht_gemm_kernel<kFourierAnalysis>, dim3(tiles(p->mm, p->nlat), a.nfield), dim3(kThreads), 0, s,
-                     a);
+hipError_t run_analysis(const SpeedyShtPlan* p, int32_t nfield, const SegTable& seg, float* y, float* modal_out,
+                        hipStream_t s) {
+  FourierArgs f = fourier_args(p, nfield, p->tl_a, seg);
+  f.out = y;
+  const FourierPair* fp = find_pair(p->n1, p->n2);
+  hipLaunchKernelGGL(fp ? fp->analysis : fourier_analysis_generic, dim3((p->nlat + p->tl_a - 1) / p->tl_a, nfield),
+                     dim3(kThreads), analysis_lds(fp != nullptr, p->nlon, p->n1, p->n2, p->tl_a), s, f);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  StageArgs a = stage_args(p, nfield);
   a.in = y;
   a.out = modal_out;
-  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreAnalysis>, dim3(tiles(a.nfield, (p->n_l + 1) / 2), 2 * p->mm),
+  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreAnalysis>, dim3(tiles(nfield, (p->n_l + 1) / 2), 2 * p->mm),
                      dim3(kThreads), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t run_synthesis(const SpeedyShtPlan* p, StageArgs a, const float* modal_in, float* y, hipStream_t s) {
+hipError_t run_synthesis(const SpeedyShtPlan* p, int32_t nfield, const SegTable& seg, const float* modal_in, float* y,
+                         hipStream_t s) {
+  StageArgs a = stage_args(p, nfield);
   a.in = modal_in;
   a.out = y;
-  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreSynthesis>, dim3(tiles(a.nfield, p->nlat / 2), 2 * p->mm),
+  hipLaunchKernelGGL(sht_gemm_kernel<kLegendreSynthesis>, dim3(tiles(nfield, p->nlat / 2), 2 * p->mm),
                      dim3(kThreads), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  a.in = y;
-  a.out = nullptr;
-  hipLaunchKernelGGL(sht_gemm_kernel<kFourierSynthesis>, dim3(tiles(p->nlon, p->nlat), a.nfield), dim3(kThreads), 0,
-                     s, a);
+  FourierArgs f = fourier_args(p, nfield, p->tl_s, seg);
+  f.in = y;
+  const FourierPair* fp = find_pair(p->n1, p->n2);
+  hipLaunchKernelGGL(fp ? fp->synthesis : fourier_synthesis_generic, dim3((p->nlat + p->tl_s - 1) / p->tl_s, nfield),
+                     dim3(kThreads), synthesis_lds(fp != nullptr, p->nlon, p->mm, p->tl_s), s, f);
   return hipGetLastError();
 }
 
@@ -609,25 +1000,44 @@ int speedy_sht_create(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, doub
   if (int rc = check_grid(n_m, n_l, nlon, nlat)) return rc;
   if (!(radius > 0.0)) return fail(SPEEDY_ERR_ARG, "speedy_sht_create: radius must be positive");
   const int mm = 2 * n_m - 1;
+  // nlon = n1 * n2 with n1 the largest divisor <= sqrt(nlon); the widest latitude
+  // tile (16, 8, .., 1) whose LDS fits
+  int n1 = 1;
+  for (int d = 1; (int64_t)d * d <= nlon; ++d)
+    if (nlon % d == 0) n1 = d;
+  const int n2 = nlon / n1;
+  // compiled pairs: a thread per (i1 or k2, latitude) in each pass, so at most
+  // kThreads / max(n1, n2) latitudes; generic kernels: 16
+  const bool fixed = find_pair(n1, n2) != nullptr;
+  int tl_a = fixed ? std::min(16, kThreads / std::max(n1, n2)) : 16, tl_s = tl_a;
+  while (tl_a > 1 && analysis_lds(fixed, nlon, n1, n2, tl_a) > kFourierLds) --tl_a;
+  while (tl_s > 1 && synthesis_lds(fixed, nlon, mm, tl_s) > kFourierLds) --tl_s;
+  if (analysis_lds(fixed, nlon, n1, n2, tl_a) > kFourierLds || synthesis_lds(fixed, nlon, mm, tl_s) > kFourierLds)
+    return fail(SPEEDY_ERR_ARG, "speedy_sht_create: nlon too large for the Fourier stages' 64 KiB of LDS");
   std::vector<double> x(nlat), w(nlat);
   gauss_nodes(nlat, x.data(), w.data());
-  std::vector<float> F((size_t)nlon * mm), P((size_t)n_m * nlat * n_l), wf(nlat), sec(nlat);
-  fill_basis(n_m, n_l, nlon, nlat, x.data(), F.data(), P.data());
+  std::vector<float> P((size_t)n_m * nlat * n_l), wf(nlat), sec(nlat), tw(2 * (size_t)nlon);
+  fill_basis(n_m, n_l, nlon, nlat, x.data(), nullptr, P.data());
   for (int j = 0; j < nlat; ++j) {
     wf[j] = (float)(w[j] * 2.0 * M_PI / nlon);
     sec[j] = (float)(1.0 / std::sqrt(1.0 - x[j] * x[j]));
   }
-  SpeedyShtPlan* p = new SpeedyShtPlan{n_m, n_l, nlon, nlat, mm, radius, 0, nullptr, nullptr, nullptr, nullptr};
+  for (int j = 0; j < nlon; ++j) {
+    tw[j] = (float)std::cos(2.0 * M_PI * j / nlon);
+    tw[nlon + j] = (float)-std::sin(2.0 * M_PI * j / nlon);
+  }
+  SpeedyShtPlan* p = new SpeedyShtPlan{n_m, n_l, nlon, nlat, mm, radius, 0, n1, n2, tl_a, tl_s,
+                                       nullptr, nullptr, nullptr, nullptr};
   hipError_t e = hipGetDevice(&p->device);
   auto up = [&](float** dst, const std::vector<float>& v) {
     if (e != hipSuccess) return;
     e = hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice);
   };
-  up(&p->F, F);
   up(&p->P, P);
   up(&p->w, wf);
   up(&p->sec, sec);
+  up(&p->tw, tw);
   if (e != hipSuccess) {
     speedy_sht_destroy(p);
     return fail(SPEEDY_ERR_HIP, std::string("speedy_sht_create: ") + hipGetErrorString(e));
@@ -638,7 +1048,7 @@ int speedy_sht_create(int32_t n_m, int32_t n_l, int32_t nlon, int32_t nlat, doub
 
 int speedy_sht_destroy(SpeedyShtPlan* p) {
   if (!p) return SPEEDY_OK;
-  for (float* q : {p->F, p->P, p->w, p->sec})
+  for (float* q : {p->P, p->w, p->sec, p->tw})
     if (q) (void)hipFree(q);
   delete p;
   return SPEEDY_OK;
@@ -652,20 +1062,20 @@ int speedy_sht_to_modal(const SpeedyShtPlan* p, const float* nodal, float* modal
                         size_t wsb, void* stream) {
   if (int rc = check_call(p, nfield, ws, wsb)) return rc;
   if (!nodal || !modal) return fail(SPEEDY_ERR_ARG, "speedy_sht_to_modal: null field");
-  StageArgs a = stage_args(p, nfield);
-  add_seg(a.seg, nodal, nullptr, nfield, 0, 1.0f);
+  SegTable seg{};
+  add_seg(seg, nodal, nullptr, nfield, 0, 1.0f);
   const Ws w = carve(p, nfield, ws);
-  return hip_result(run_analysis(p, a, w.y, modal, static_cast<hipStream_t>(stream)));
+  return hip_result(run_analysis(p, nfield, seg, w.y, modal, static_cast<hipStream_t>(stream)));
 }
 
 int speedy_sht_to_nodal(const SpeedyShtPlan* p, const float* modal, float* nodal, int32_t nfield, void* ws,
                         size_t wsb, void* stream) {
   if (int rc = check_call(p, nfield, ws, wsb)) return rc;
   if (!nodal || !modal) return fail(SPEEDY_ERR_ARG, "speedy_sht_to_nodal: null field");
-  StageArgs a = stage_args(p, nfield);
-  add_seg(a.seg, nullptr, nodal, nfield, 0, 1.0f);
+  SegTable seg{};
+  add_seg(seg, nullptr, nodal, nfield, 0, 1.0f);
   const Ws w = carve(p, nfield, ws);
-  return hip_result(run_synthesis(p, a, modal, w.y, static_cast<hipStream_t>(stream)));
+  return hip_result(run_synthesis(p, nfield, seg, modal, w.y, static_cast<hipStream_t>(stream)));
 }
 
 int speedy_dynamics_to_physics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec* spec, const SpeedySpectralState* in,
@@ -711,14 +1121,14 @@ int speedy_dynamics_to_physics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec*
   hipLaunchKernelGGL(modal_winds_kernel, dim3(blocks(per * (nfield - kx))), dim3(kThreads), 0, s, m);
   if (int rc = hip_result(hipGetLastError())) return rc;
   // 2. synthesis with the nodal post-operations
-  StageArgs a = stage_args(p, nfield);
-  add_seg(a.seg, nullptr, out->u_wind, kx, kOpSecLat, 1.0f);
-  add_seg(a.seg, nullptr, out->v_wind, kx, kOpSecLat, 1.0f);
-  add_seg(a.seg, nullptr, out->temperature, kx, kOpAddLevel, 1.0f, spec->reference_temperature, kx);
-  add_seg(a.seg, nullptr, out->specific_humidity, kx, 0, 1000.0f);
-  add_seg(a.seg, nullptr, out->normalized_surface_pressure, nsp, kOpExp, 1.0f);
-  add_seg(a.seg, nullptr, out->geopotential, kx, 0, 1.0f);
-  return hip_result(run_synthesis(p, a, w.z, w.y, s));
+  SegTable seg{};
+  add_seg(seg, nullptr, out->u_wind, kx, kOpSecLat, 1.0f);
+  add_seg(seg, nullptr, out->v_wind, kx, kOpSecLat, 1.0f);
+  add_seg(seg, nullptr, out->temperature, kx, kOpAddLevel, 1.0f, spec->reference_temperature, kx);
+  add_seg(seg, nullptr, out->specific_humidity, kx, 0, 1000.0f);
+  add_seg(seg, nullptr, out->normalized_surface_pressure, nsp, kOpExp, 1.0f);
+  add_seg(seg, nullptr, out->geopotential, kx, 0, 1.0f);
+  return hip_result(run_synthesis(p, nfield, seg, w.z, w.y, s));
 }
 
 namespace {
@@ -738,16 +1148,16 @@ int nodal_to_spectral(const SpeedyShtPlan* p, const SpeedyDynamicsSpec* spec, co
   if (int rc = check_call(p, nfield, ws, wsb)) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const Ws w = carve(p, nfield, ws);
-  StageArgs a = stage_args(p, nfield);
-  add_seg(a.seg, in->u_wind, nullptr, kx, kOpSecLat, 1.0f);
-  add_seg(a.seg, in->v_wind, nullptr, kx, kOpSecLat, 1.0f);
+  SegTable seg{};
+  add_seg(seg, in->u_wind, nullptr, kx, kOpSecLat, 1.0f);
+  add_seg(seg, in->v_wind, nullptr, kx, kOpSecLat, 1.0f);
   if (tendency)
-    add_seg(a.seg, in->temperature, nullptr, kx, 0, 1.0f);
+    add_seg(seg, in->temperature, nullptr, kx, 0, 1.0f);
   else
-    add_seg(a.seg, in->temperature, nullptr, kx, kOpSubLevel, 1.0f, spec->reference_temperature, kx);
-  add_seg(a.seg, in->specific_humidity, nullptr, kx, 0, 1e-3f);
-  if (!tendency) add_seg(a.seg, in->normalized_surface_pressure, nullptr, nsp, kOpLog, 1.0f);
-  if (int rc = hip_result(run_analysis(p, a, w.y, w.z, s))) return rc;
+    add_seg(seg, in->temperature, nullptr, kx, kOpSubLevel, 1.0f, spec->reference_temperature, kx);
+  add_seg(seg, in->specific_humidity, nullptr, kx, 0, 1e-3f);
+  if (!tendency) add_seg(seg, in->normalized_surface_pressure, nullptr, nsp, kOpLog, 1.0f);
+  if (int rc = hip_result(run_analysis(p, nfield, seg, w.y, w.z, s))) return rc;
   ModalArgs m{};
   m.mm = p->mm; m.nl = p->n_l; m.kx = kx;
   m.inv_radius = (float)(1.0 / p->radius);

@@ -2,10 +2,11 @@
 the float64 oracle (oracle/sht_oracle.py), the closed forms the mathematics
 fixes, and the reference's own round trip (physics_interface_test.py:9-44).
 
-Tolerance: the GPU computes in float32 (fp32 MFMA, sums of up to 1280
-products per stage) against a float64 oracle, so each field is compared by
-max|gpu - oracle| / max|oracle| <= TOL (TOL = 5e-5; float32's epsilon is 6e-8
-and the longest contraction here has 256 terms per stage at T85)."""
+Tolerance: the GPU computes in float32 (fp32 MFMA GEMMs for the Legendre
+stages, sums of up to 427 products; two short DFT passes of n1 and n2 terms for
+the Fourier stages, nlon = n1 * n2) against a float64 oracle, so each field is
+compared by max|gpu - oracle| / max|oracle| <= TOL (TOL = 5e-5; float32's
+epsilon is 6e-8)."""
 import numpy as np
 import pytest
 
@@ -73,6 +74,36 @@ def test_to_nodal_and_to_modal_match_the_oracle(tr, trunc):
     assert _rel(gg.to_modal(torch.as_tensor(x, device="cuda")), og.to_modal(x.astype(np.float64))) <= TOL
     # entries with m > l come out exactly zero
     assert not modal.cpu().numpy()[:, ~og.mask].any()
+
+
+@pytest.mark.parametrize("dims", [(22, 23, 66, 32), (30, 31, 97, 48)])
+def test_generic_fourier_stages_match_the_oracle(tr, dims):
+    """Grids whose nlon is not one of the compiled factor pairs (66 = 6 x 11;
+    97, prime: a single 97-point pass) take the generic LDS form of the two
+    Fourier passes: the same checks as above against the oracle on that grid."""
+    import torch
+    gg, og = tr.SpectralGrid(*dims), so.Grid(*dims)
+    rng = np.random.default_rng(dims[2])
+    c = _random_modal(og, (4,), rng).astype(np.float32)
+    nodal = gg.to_nodal(torch.as_tensor(c, device="cuda"))
+    assert _rel(nodal, og.to_nodal(c.astype(np.float64))) <= TOL
+    assert _rel(gg.to_modal(nodal), c) <= TOL
+    x = rng.standard_normal((3,) + og.nodal_shape).astype(np.float32)
+    assert _rel(gg.to_modal(torch.as_tensor(x, device="cuda")), og.to_modal(x.astype(np.float64))) <= TOL
+
+
+def test_t425_round_trip(tr):
+    """T425 (1280 x 640; its Fourier stages take the generic form, 4 latitudes a
+    workgroup): a band-limited field survives to_nodal then to_modal, which the
+    Gaussian quadrature and the DFT make exact up to float32 rounding."""
+    import torch
+    gg = tr.SpectralGrid.truncation(425)   # no float64 oracle at this size: its bases would take ~1 GB
+    mode_m = np.array([0] + [m for mm in range(1, gg.n_m) for m in (mm, mm)])
+    mask = mode_m[:, None] <= np.arange(gg.n_l)[None, :]
+    rng = np.random.default_rng(425)
+    c = (rng.standard_normal((2,) + mask.shape) * mask).astype(np.float32)
+    back = gg.to_modal(gg.to_nodal(torch.as_tensor(c, device="cuda")))
+    assert _rel(back, c) <= TOL
 
 
 def test_repeat_calls_are_bit_identical(tr):
