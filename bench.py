@@ -450,8 +450,12 @@ def main():
     case = to_device(case_np, dev)
     kx, ix, il = case_np[2].nodal_shape
     ncol = ix * il
-    phys = SpeedyPhysics(device=dev)
-    vphys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2), device=dev)
+    # The profile run launches every call plainly (as graph replays do): under
+    # rocprofv3, a process that made cooperative launches ends in the tool's exit
+    # handlers with SIGSEGV after the results are written.
+    plain = args.profile_only
+    phys = SpeedyPhysics(device=dev, plain_launch=plain)
+    vphys = SpeedyPhysics(parameters=synthetic.draw_vjp_parameters(2), device=dev, plain_launch=plain)
     gen = torch.Generator(device=dev)
     gen.manual_seed(2)
     cot = PhysicsTendency(*[torch.randn(case[0].temperature.shape, device=dev, generator=gen) for _ in range(4)])
