@@ -169,11 +169,46 @@ __device__ __forceinline__ void store_uv_zeros(const KArgs& a, uint32_t m, uint3
   }
 }
 
+// Up to eight levels, the term-ordered tendency sums (speedy_physics.py:123-125)
+// are formed as early as their terms exist, in the same order of float
+// additions: convection and condensation right after phase A; the q tendency of
+// every level but the lowest once vertical diffusion (its last term there) is
+// known.  The phase-A and vertical-diffusion tendencies then die early instead of
+// staying live across phases b1 and b2 (16 fewer registers in the forward, and
+// no scratch at two waves per SIMD).  The sixteen-level kernels keep the sums at
+// the stores: they spill, and the early form gave wrong 16-level fused forward
+// tendencies on gfx950 (the host build of the same order is right).
+template <int KX>
+constexpr bool kEarlySums = KX <= 8;
+
+template <int KX>
+__device__ __forceinline__ void tend_after_a(uint32_t tm, const PhaseA<float, KX>& pa, float at[KX], float aq[KX]) {
+  if constexpr (!kEarlySums<KX>) return;
+#pragma unroll
+  for (int k = 0; k < KX; ++k) {
+    float t = 0.0f, q = 0.0f;
+    if (tm & (1u << SPEEDY_TERM_CONVECTION)) { t = t + pa.tconv[k]; q = q + pa.qconv[k]; }
+    if (tm & (1u << SPEEDY_TERM_CONDENSATION)) { t = t + pa.tlsc[k]; q = q + pa.qlsc[k]; }
+    at[k] = t;
+    aq[k] = q;
+  }
+}
+template <int KX>
+__device__ __forceinline__ void tend_after_b1(uint32_t tm, const PhaseB<float, KX>& pb, float aq[KX]) {
+  if constexpr (!kEarlySums<KX>) return;
+  if (tm & (1u << SPEEDY_TERM_VDIFF)) {
+#pragma unroll
+    for (int k = 0; k < KX - 1; ++k) aq[k] = aq[k] + pb.qvd[k];
+  }
+}
+
 // Returns the levels whose q tendency verify_tendencies limited (bit k).
+// at, aq: the partial sums of tend_after_a / tend_after_b1.
 template <int KX, bool DIAG, bool EXTRA, bool UV_ALL>
 __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, uint32_t c,
                                                   const ColumnIn<float, KX>& col, const PhaseA<float, KX>& pa,
-                                                  const PhaseB<float, KX>& pb, double colsum[2]) {
+                                                  const PhaseB<float, KX>& pb, const float at[KX], const float aq[KX],
+                                                  double colsum[2]) {
   uint32_t limited = 0u;
   const bool budget = EXTRA && a.f.budget != nullptr;
   const double dpg = (double)col.ps * (double)(P0 / GRAV);   // p0 ps / g; times dhs_k: dp_k / g
@@ -186,14 +221,22 @@ __device__ __forceinline__ uint32_t store_outputs(const KArgs& a, uint32_t m, ui
 #pragma unroll
   for (int k = 0; k < KX; ++k) {
     float t = 0.0f, q = 0.0f, du = 0.0f, dv = 0.0f;
-    if (tm & (1u << SPEEDY_TERM_CONVECTION)) { t = t + pa.tconv[k]; q = q + pa.qconv[k]; }
-    if (tm & (1u << SPEEDY_TERM_CONDENSATION)) { t = t + pa.tlsc[k]; q = q + pa.qlsc[k]; }
+    if (kEarlySums<KX>) {
+      t = at[k];
+      q = aq[k];
+    } else {
+      if (tm & (1u << SPEEDY_TERM_CONVECTION)) { t = t + pa.tconv[k]; q = q + pa.qconv[k]; }
+      if (tm & (1u << SPEEDY_TERM_CONDENSATION)) { t = t + pa.tlsc[k]; q = q + pa.qlsc[k]; }
+    }
     if (tm & (1u << SPEEDY_TERM_SHORTWAVE)) t = t + pb.tsw[k];
     if (k == KX - 1 && (tm & (1u << SPEEDY_TERM_SURFACE))) {
       du = du + pb.du1; dv = dv + pb.dv1; t = t + pb.dts1; q = q + pb.dqs1;
     }
     if (tm & (1u << SPEEDY_TERM_LW_UP)) t = t + pb.tlw[k];
-    if (tm & (1u << SPEEDY_TERM_VDIFF)) { t = t + pb.tvd[k]; q = q + pb.qvd[k]; }
+    if (tm & (1u << SPEEDY_TERM_VDIFF)) {
+      t = t + pb.tvd[k];
+      if (!kEarlySums<KX> || k == KX - 1) q = q + pb.qvd[k];   // early sums: the levels above took it already
+    }
     if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_TENDENCIES)) {
       bool lim;
       q = verify_dq(gld(a.f.q, base3 + k * ncol), q, a.u.dt_step, lim);   // the unclamped q (re-read, L2)
@@ -574,6 +617,8 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   }
   PhaseA<float, KX> pa;
   phase_a<float, KX>(col, a.p, a.v, a.u, pa);
+  float tend_t[KX], tend_q[KX];
+  tend_after_a<KX>(a.term_mask, pa, tend_t, tend_q);
   if (FUSED && need_s) {
     // publish this block's word(s) for the call's level set: a wave-level OR per
     // member slot (a fused block spans at most two members), then one word per slot
@@ -593,6 +638,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   }
   PhaseB<float, KX> pb;
   phase_b1<float, KX>(col, pa, a.p, a.v, a.u, pb);
+  tend_after_b1<KX>(a.term_mask, pb, tend_q);
   uint32_t S = (uint32_t)a.level_set;
   if (VJP && need_s) {
     // the forward + VJP kernels: wave 0 gathers S for the block's member(s)
@@ -665,7 +711,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
   uint32_t limited = 0u;
   double colsum[2] = {0.0, 0.0};
-  if (active) limited = store_outputs<KX, DIAG, EXTRA, VJP>(a, m, c, col, pa, pb, colsum);
+  if (active) limited = store_outputs<KX, DIAG, EXTRA, VJP>(a, m, c, col, pa, pb, tend_t, tend_q, colsum);
   if (EXTRA && a.f.budget) {   // float64 column budgets (block-uniform branch)
     const uint32_t j = c - divq(c, a.dlat) * a.nlat;
     const double w = (double)gld(a.f.coa, j);
@@ -826,8 +872,8 @@ __global__ __launch_bounds__(kBlock, KX <= 8 ? 2 : 1) void fwd_only_batch_kernel
 // Forward-only segments of a batch in which none needs the verify steps or the
 // float64 budgets: the forward without that optional code (4 % faster at T425).
 // Both forward-only batch kernels are built for two waves per SIMD at 7 and 8
-// levels (at most 256 registers a lane; 20-44 B of scratch a lane at 8): the
-// second wave covers the first one's stalls, 128 -> 86 us at T425 x 8 levels.
+// levels (at most 256 registers a lane, no scratch): the second wave covers the
+// first one's stalls, 128 -> 86 us at T425 x 8 levels.
 template <int KX>
 __global__ __launch_bounds__(kBlock, KX <= 8 ? 2 : 1) void fwd_plain_batch_kernel(const KArgs* __restrict__ segs,
                                                                  const uint32_t* __restrict__ table) {
@@ -845,7 +891,7 @@ __global__ __launch_bounds__(kBlock) void fwd_diag_batch_kernel(const KArgs* __r
 }
 
 // Forward + VJP segments of a batch.  Built for two waves per SIMD at 7 and 8
-// levels (at most 256 registers a lane, 40-152 B of scratch a lane, 37 KB of LDS
+// levels (at most 256 registers a lane, 36-148 B of scratch a lane, 37 KB of LDS
 // a block): 521 -> 448 us at T425 x 8 levels, against one wave per SIMD without
 // scratch.  The fused forward + VJP kernel keeps one wave per SIMD and no scratch.
 template <int KX>
