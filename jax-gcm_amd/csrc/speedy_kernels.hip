@@ -727,16 +727,29 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
     verify_ct<float, KX>(EXTRA ? a.flags : 0u, limited, a.u.dt_step, ct, g_qraw);
     ColumnAdj<float, KX> g;
     // longwave flux tapes in LDS, [slot][thread]: the kernel runs one block per CU
-    __shared__ float s_tape[vjp_tape_floats<KX>() * kBlock];
+    // (up to eight levels, KX more slots hold g_qraw across the adjoint, where it
+    // would otherwise take KX registers)
+    constexpr int kQraw = KX <= 8 ? KX : 0;
+    __shared__ float s_tape[(vjp_tape_floats<KX>() + kQraw) * kBlock];
     LdsStore tape{s_tape + threadIdx.x};
+    if constexpr (kQraw > 0) {
+#pragma unroll
+      for (int k = 0; k < KX; ++k) tape.put(vjp_tape_floats<KX>() + k, g_qraw[k]);
+    }
     column_vjp<float, KX>(col, pa, pb, S, a.p, a.v, a.u, a.flags, ct, g, tape);
     if (EXTRA && (a.flags & SPEEDY_FLAG_VERIFY_STATE)) {
       const size_t base3 = (size_t)m * KX * a.ncol + c;
 #pragma unroll
       for (int k = 0; k < KX; ++k) g.Q[k] = verify_q_grad(gld(a.f.q, base3 + k * (size_t)a.ncol), g.Q[k]);
     }
+    if constexpr (kQraw > 0) {
+      const LdsStore t = tape.after(g.Q[0]);
 #pragma unroll
-    for (int k = 0; k < KX; ++k) g.Q[k] += g_qraw[k];
+      for (int k = 0; k < KX; ++k) g.Q[k] += t.get(vjp_tape_floats<KX>() + k);
+    } else {
+#pragma unroll
+      for (int k = 0; k < KX; ++k) g.Q[k] += g_qraw[k];
+    }
     if (active) store_grads<KX>(a, m, c, g);
     block_param_partials(a, bid, gid, active, m - m_first, g.prm);
   }
@@ -891,9 +904,10 @@ __global__ __launch_bounds__(kBlock) void fwd_diag_batch_kernel(const KArgs* __r
 }
 
 // Forward + VJP segments of a batch.  Built for two waves per SIMD at 7 and 8
-// levels (at most 256 registers a lane, 36-148 B of scratch a lane, 37 KB of LDS
-// a block): 521 -> 448 us at T425 x 8 levels, against one wave per SIMD without
-// scratch.  The fused forward + VJP kernel keeps one wave per SIMD and no scratch.
+// levels (at most 256 registers a lane, 132 B of scratch a lane at 8 levels and
+// none at 7, 45 KB of LDS a block): 521 -> 412 us at T425 x 8 levels, against one
+// wave per SIMD without scratch.  The fused forward + VJP kernel keeps one wave
+// per SIMD and no scratch.
 template <int KX>
 __global__ __launch_bounds__(kBlock, KX <= 8 ? 2 : 1) void fwd_batch_kernel(const KArgs* segs, const uint32_t* table) {
   const uint32_t e = __builtin_amdgcn_readfirstlane(table[blockIdx.x]);

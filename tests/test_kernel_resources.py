@@ -10,9 +10,9 @@ and 8 levels fit in registers too.
 
 The batched kernels at 7 and 8 levels (forward-only and forward + VJP) are
 built for two waves per SIMD instead, at most 256 registers a lane: the
-forward-only ones without scratch, the forward + VJP one with some (148 B a
-lane at 8 levels), which makes it 1.16x faster at T425 than one wave per SIMD
-without scratch.  Their register count and scratch are bounded here.  A change that brings scratch back or raises it fails
+forward-only ones without scratch, the forward + VJP one with some (132 B a
+lane at 8 levels, none at 7), which makes it 1.26x faster at T425 than one
+wave per SIMD without scratch.  Their register count and scratch are bounded here.  A change that brings scratch back or raises it fails
 here, on the CPU, before any GPU run.  (Sixteen-level kernels still spill; they
 are not checked.)
 """
