@@ -58,7 +58,9 @@ def _dynamics(tr, gg, og, kx, rng):
     return dyn_gpu, dyn_ref
 
 
-@pytest.mark.parametrize("trunc", [21, 31, 85, 170])
+# every compiled Fourier factor pair: 64 = 8x8, 96 = 8x12, 128 = 8x16 (T42), 256 = 16x16,
+# 320 = 16x20, 360 = 18x20, 512 = 16x32, 640 = 20x32
+@pytest.mark.parametrize("trunc", [21, 31, 42, 85, 106, 119, 170, 213])
 def test_to_nodal_and_to_modal_match_the_oracle(tr, trunc):
     import torch
     gg, og = _grids(tr, trunc)
