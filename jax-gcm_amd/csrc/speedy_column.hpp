@@ -715,13 +715,16 @@ struct LwLevel {
   R tau[4], st0, st1, fb[4];
 };
 
-template <typename R, int KX>
+// TAU = false: the emissions and band fractions only (the caller has tau).
+template <typename R, int KX, bool TAU = true>
 SPD_HD void lw_level(int k, const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
                      const SpeedyVertical& v, const Uniforms<R>& u, LwLevel<R>& L) {
-  R ab[4];
-  lw_absorptivity<R, KX>(k, c, a, p, u, ab);
+  if constexpr (TAU) {
+    R ab[4];
+    lw_absorptivity<R, KX>(k, c, a, p, u, ab);
 #pragma unroll
-  for (int bnd = 0; bnd < 4; ++bnd) L.tau[bnd] = exp_tr(-ab[bnd] * c.ps * v.dhs[k]);
+    for (int bnd = 0; bnd < 4; ++bnd) L.tau[bnd] = exp_tr(-ab[bnd] * c.ps * v.dhs[k]);
+  }
   auto Th = [&](int j) { return c.T[j] + u.wvi1[j] * (c.T[j + 1] - c.T[j]); };
   if (k == 0) {
     const R s0 = R(0.75) * c.T[0] + R(0.25) * Th(0);
@@ -863,11 +866,29 @@ SPD_HD void vdiff(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const Speedy
   for (int k = 0; k < KX; ++k) { b.tvd[k] = tt[k]; b.qvd[k] = qt[k]; }
 }
 
-template <typename R, int KX>
+// Where the longwave transmissivities wait between longwave down (phase b1) and
+// longwave up (phase b2): nowhere by default (longwave up recomputes them per
+// level); the forward kernels pass a per-column LDS store (put / get / after).
+struct NoTauStore {
+  static constexpr bool kOn = false;
+  template <typename T>
+  SPD_HD void put(int, T) const {}
+  SPD_HD float get(int) const { return 0.0f; }
+  template <typename T>
+  SPD_HD NoTauStore after(T) const { return *this; }
+};
+
+template <typename R, int KX, class TauStore = NoTauStore>
 SPD_HD void phase_b1(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p,
-                     const SpeedyVertical& v, const Uniforms<R>& u, PhaseB<R, KX>& b) {
+                     const SpeedyVertical& v, const Uniforms<R>& u, PhaseB<R, KX>& b, TauStore ts = TauStore()) {
   albedos<R, KX>(c, p, b);
   lw_transmissivity<R, KX>(c, a, p, v, u, b);
+  if constexpr (TauStore::kOn) {
+#pragma unroll
+    for (int k = 0; k < KX; ++k)
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) ts.put(k * 4 + bnd, b.tauL[k][bnd]);
+  }
   LwdTape<R, KX> ld;
   lw_down<R, KX>(c, p, u, b, ld);
   vdiff<R, KX>(c, a, p, u, b);
@@ -1101,9 +1122,22 @@ struct LwuTape {
 // are recomputed (lw_level, equal to lw_transmissivity's and lw_emissions'), so
 // the [KX][4] arrays of phase b1 are dead after longwave down and do not stay
 // live across the shortwave and the surface fluxes.
-template <typename R, int KX, bool DIAG>
+template <typename R, int KX, bool DIAG, class TauStore = NoTauStore>
 SPD_HD void lw_up(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const SpeedyParams& p, const SpeedyVertical& v,
-                  const Uniforms<R>& u, PhaseB<R, KX>& b, R rlus2, R tsfc, const R flux_h[4], LwuTape<R, KX>& t) {
+                  const Uniforms<R>& u, PhaseB<R, KX>& b, R rlus2, R tsfc, const R flux_h[4], LwuTape<R, KX>& t,
+                  TauStore ts = TauStore()) {
+  // level k's longwave quantities: the transmissivities from the store when
+  // there is one (each level read after the previous level's flux exists)
+  auto level = [&](int k, R dep, LwLevel<R>& L) {
+    if constexpr (TauStore::kOn) {
+      lw_level<R, KX, false>(k, c, a, p, v, u, L);
+      const auto s = ts.after(dep);
+#pragma unroll
+      for (int bnd = 0; bnd < 4; ++bnd) L.tau[bnd] = s.get(k * 4 + bnd);
+    } else {
+      lw_level<R, KX>(k, c, a, p, v, u, L);
+    }
+  };
   const R ps = c.ps;
   const R rps = mdiv(R(1.0), ps);
   const R stratc0 = c.stratz * ps, stratc1 = u.eps1 * ps;
@@ -1122,7 +1156,7 @@ SPD_HD void lw_up(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const Speedy
 #pragma unroll
   for (int k = KX - 1; k >= 1; --k) {
     LwLevel<R> L;
-    lw_level<R, KX>(k, c, a, p, v, u, L);
+    level(k, Fu[0], L);
     if (k == 1) st4a0_1 = L.st0;
 #pragma unroll
     for (int bnd = 0; bnd < 4; ++bnd) {
@@ -1136,7 +1170,7 @@ SPD_HD void lw_up(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const Speedy
   }
   R ft[4];
   LwLevel<R> L0;
-  lw_level<R, KX>(0, c, a, p, v, u, L0);
+  level(0, Fu[0], L0);
 #pragma unroll
   for (int bnd = 0; bnd < 2; ++bnd) {
     const R em = R(1.0) - L0.tau[bnd];
@@ -1163,9 +1197,10 @@ SPD_HD void lw_up(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, const Speedy
   }
 }
 
-template <typename R, int KX, bool DIAG>
+template <typename R, int KX, bool DIAG, class TauStore = NoTauStore>
 SPD_HD void phase_b2(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, uint32_t S, const SpeedyParams& p,
-                     const SpeedyVertical& v, const Uniforms<R>& u, uint32_t flags, PhaseB<R, KX>& b) {
+                     const SpeedyVertical& v, const Uniforms<R>& u, uint32_t flags, PhaseB<R, KX>& b,
+                     TauStore ts = TauStore()) {
   R flux_h[4];
   SwTape<R, KX> sw;
   shortwave<R, KX, DIAG>(c, a, S, p, v, u, b, flux_h, sw);
@@ -1174,7 +1209,7 @@ SPD_HD void phase_b2(const ColumnIn<R, KX>& c, const PhaseA<R, KX>& a, uint32_t 
   surface<R, KX, DIAG>(c, a, p, v, u, flags, b, tsfc, sf);
   b.flux_h[0] = flux_h[0]; b.flux_h[1] = flux_h[1]; b.rlus2 = sf.rlus2; b.tsfc = tsfc;
   LwuTape<R, KX> lu;
-  lw_up<R, KX, DIAG>(c, a, p, v, u, b, sf.rlus2, tsfc, flux_h, lu);
+  lw_up<R, KX, DIAG, TauStore>(c, a, p, v, u, b, sf.rlus2, tsfc, flux_h, lu, ts);
 }
 
 }  // namespace speedy

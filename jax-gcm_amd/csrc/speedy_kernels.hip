@@ -434,6 +434,7 @@ __device__ __forceinline__ void levelset_body(const KArgs& a, uint32_t bid, uint
 // ---------------------------------------------------------------- VJP helpers
 // One column's slice of an LDS array laid out [slot][thread] (conflict-free).
 struct LdsStore {
+  static constexpr bool kOn = true;
   float* base;
   int off = 0;
   __device__ __forceinline__ void put(int i, float x) { base[i * kBlock + off] = x; }
@@ -577,9 +578,28 @@ __device__ __forceinline__ void block_budget_partials(const KArgs& a, uint32_t b
 // level-set kernel of the same batch.
 // EXTRA: the optional verify steps and float64 budgets are compiled in (their
 // flags and pointers are then read at run time); the plain forward omits them.
+// The batched forward kernels up to eight levels keep the longwave
+// transmissivities in LDS between longwave down and longwave up (4 KX floats a
+// column, [slot][thread]) instead of recomputing them per level (T425: 153 fewer
+// VALU instructions a wave, ~1 % faster).  The fused kernels recompute: at one
+// wave per SIMD the dependent LDS reads in the longwave-up sweep cost more than
+// the exponentials (T85 forward ~2 % slower with the store).  The forward + VJP
+// kernels' LDS holds the adjoint's tapes, and the sixteen-level kernels keep their
+// tested code.
+template <bool ON>
+__device__ __forceinline__ auto tau_store(float* s) {
+  if constexpr (ON)
+    return LdsStore{s + threadIdx.x};
+  else
+    return NoTauStore{};
+}
+
 template <int KX, bool DIAG, bool FUSED, bool VJP, bool EXTRA>
 __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   __shared__ uint32_t s_s[2];
+  constexpr bool kTau = !VJP && !FUSED && KX <= 8;
+  __shared__ float s_tau[kTau ? 4 * KX * kBlock : 1];
+  const auto ts = tau_store<kTau>(s_tau);
   const uint32_t gid = bid * kBlock + threadIdx.x;
   const bool active = gid < a.ntot;
   const uint32_t g0 = gid < a.ntot ? gid : a.ntot - 1u;   // inactive lanes shadow a valid column
@@ -637,7 +657,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
     }
   }
   PhaseB<float, KX> pb;
-  phase_b1<float, KX>(col, pa, a.p, a.v, a.u, pb);
+  phase_b1<float, KX>(col, pa, a.p, a.v, a.u, pb, ts);
   tend_after_b1<KX>(a.term_mask, pb, tend_q);
   uint32_t S = (uint32_t)a.level_set;
   if (VJP && need_s) {
@@ -708,7 +728,7 @@ __device__ __forceinline__ void fwd_body(const KArgs& a, uint32_t bid) {
   // moving them earlier changed their register allocation and, at 16 levels,
   // gave wrong gradients on gfx950 (the kernels spill ~4 KB per lane).
   if (VJP) load_zonal<KX>(a, c, col);
-  phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb);
+  phase_b2<float, KX, DIAG>(col, pa, S, a.p, a.v, a.u, a.flags, pb, ts);
   uint32_t limited = 0u;
   double colsum[2] = {0.0, 0.0};
   if (active) limited = store_outputs<KX, DIAG, EXTRA, VJP>(a, m, c, col, pa, pb, tend_t, tend_q, colsum);
