@@ -678,13 +678,17 @@ __device__ __forceinline__ float d_dlon(const float* x, int mode, int m, int l, 
 
 // Modal (psi, chi) -> (u cos(lat), v cos(lat)) for every level, then copies of
 // the remaining modal inputs, into one contiguous stack (vor_div_to_uv_nodal's
-// modal half and compute_diagnostic_state's inputs).
+// modal half and compute_diagnostic_state's inputs).  blockIdx.y is the row:
+// rows below kx are the winds of level blockIdx.y (writing the u and v rows),
+// the others write output row blockIdx.y + kx; blockIdx.x covers the row's
+// mm x nl coefficients.
 __global__ __launch_bounds__(kThreads) void modal_winds_kernel(ModalArgs a) {
   const int per = a.mm * a.nl;
-  int idx = blockIdx.x * kThreads + threadIdx.x;
-  const int nwind = a.kx * per;
-  if (idx < nwind) {
-    const int k = idx / per, rem = idx - k * per, mode = rem / a.nl, l = rem - mode * a.nl;
+  const int rem = blockIdx.x * kThreads + threadIdx.x;
+  if (rem >= per) return;
+  int r = blockIdx.y;
+  if (r < a.kx) {
+    const int k = r, mode = rem / a.nl, l = rem - mode * a.nl;
     const int m = (mode + 1) >> 1;
     const float* z = a.vor + (size_t)k * per;
     const float* d = a.div + (size_t)k * per;
@@ -713,39 +717,47 @@ __global__ __launch_bounds__(kThreads) void modal_winds_kernel(ModalArgs a) {
     a.out[(size_t)(a.kx + k) * per + rem] = v;
     return;
   }
-  idx -= nwind;
-  size_t row = 2 * (size_t)a.kx;
+  float* dst = a.out + (size_t)(blockIdx.y + a.kx) * per + rem;
+  r -= a.kx;
   for (int s = 0; s < 4; ++s) {
-    const int n = a.copy_count[s] * per;
-    if (idx < n) {
-      a.out[row * per + idx] = a.copy_src[s] ? a.copy_src[s][idx] : 0.0f;
+    if (r < a.copy_count[s]) {
+      *dst = a.copy_src[s] ? a.copy_src[s][(size_t)r * per + rem] : 0.0f;
       return;
     }
-    idx -= n;
-    row += a.copy_count[s];
+    r -= a.copy_count[s];
   }
-  if (!a.geo || idx >= a.kx * per) return;
+  if (!a.geo || r >= a.kx) return;
   // get_geopotential in modal space: the reference temperature enters the
-  // (mode 0, l 0) coefficient only
-  const int j = idx / per, rem = idx - j * per;
+  // (mode 0, l 0) coefficient only.  Every level's load is issued before the
+  // sum (levels past kx re-read the last one and are not summed).
+  const int j = r;
+  float t[SPEEDY_SHT_KX_MAX];
+#pragma unroll
+  for (int k = 0; k < SPEEDY_SHT_KX_MAX; ++k) t[k] = a.tvar[(size_t)min(k, a.kx - 1) * per + rem];
   float acc = a.gravity * a.orography[rem];
-  for (int k = 0; k < a.kx; ++k) {
-    float tk = a.tvar[(size_t)k * per + rem];
-    if (rem == 0) tk += a.reference[k] * kConstCoef;
-    acc += a.geo_w[j][k] * tk;
+#pragma unroll
+  for (int k = 0; k < SPEEDY_SHT_KX_MAX; ++k) {
+    if (k < a.kx) {
+      float tk = t[k];
+      if (rem == 0) tk += a.reference[k] * kConstCoef;
+      acc += a.geo_w[j][k] * tk;
+    }
   }
-  a.out[(row + j) * per + rem] = acc;
+  *dst = acc;
 }
 
 // Modal (U, V) = (u, v)/cos(lat) -> vorticity and divergence (curl_cos_lat,
 // div_cos_lat, clipped), then the remaining stacked rows to their destinations
-// (null source rows: zeros).
+// (null source rows: zeros).  blockIdx.y is the row as in modal_winds_kernel:
+// rows below kx are the winds of that level, the others copy stacked row
+// blockIdx.y + kx.
 __global__ __launch_bounds__(kThreads) void modal_vordiv_kernel(ModalArgs a) {
   const int per = a.mm * a.nl;
-  int idx = blockIdx.x * kThreads + threadIdx.x;
-  const int nwind = a.kx * per;
-  if (idx < nwind) {
-    const int k = idx / per, rem = idx - k * per, mode = rem / a.nl, l = rem - mode * a.nl;
+  const int rem = blockIdx.x * kThreads + threadIdx.x;
+  if (rem >= per) return;
+  int r = blockIdx.y;
+  if (r < a.kx) {
+    const int k = r, mode = rem / a.nl, l = rem - mode * a.nl;
     const int m = (mode + 1) >> 1;
     const float* U = a.out + (size_t)k * per;
     const float* V = a.out + (size_t)(a.kx + k) * per;
@@ -770,16 +782,14 @@ __global__ __launch_bounds__(kThreads) void modal_vordiv_kernel(ModalArgs a) {
     a.div_out[(size_t)k * per + rem] = d;
     return;
   }
-  idx -= nwind;
-  size_t row = 2 * (size_t)a.kx;
+  const float* src = a.out + (size_t)(blockIdx.y + a.kx) * per + rem;
+  r -= a.kx;
   for (int s = 0; s < 4; ++s) {
-    const int n = a.copy_count[s] * per;
-    if (idx < n) {
-      a.copy_dst[s][idx] = a.copy_src[s] ? a.out[row * per + idx] : 0.0f;
+    if (r < a.copy_count[s]) {
+      a.copy_dst[s][(size_t)r * per + rem] = a.copy_src[s] ? *src : 0.0f;
       return;
     }
-    idx -= n;
-    row += a.copy_count[s];
+    r -= a.copy_count[s];
   }
 }
 
@@ -1129,7 +1139,7 @@ int speedy_dynamics_to_physics(const SpeedyShtPlan* p, const SpeedyDynamicsSpec*
     for (int k = 0; k < kx; ++k)
       m.geo_w[j][k] = (float)(spec->gas_constant * (k < j ? 0.0 : k == j ? alpha[j] : alpha[k] + alpha[k - 1]));
   const size_t per = (size_t)p->mm * p->n_l;
-  hipLaunchKernelGGL(modal_winds_kernel, dim3(blocks(per * (nfield - kx))), dim3(kThreads), 0, s, m);
+  hipLaunchKernelGGL(modal_winds_kernel, dim3(blocks(per), nfield - kx), dim3(kThreads), 0, s, m);
   if (int rc = hip_result(hipGetLastError())) return rc;
   // 2. synthesis with the nodal post-operations
   SegTable seg{};
@@ -1179,7 +1189,7 @@ int nodal_to_spectral(const SpeedyShtPlan* p, const SpeedyDynamicsSpec* spec, co
   m.copy_src[1] = w.z; m.copy_count[1] = kx; m.copy_dst[1] = out->specific_humidity;
   m.copy_src[2] = tendency ? nullptr : w.z; m.copy_count[2] = nsp; m.copy_dst[2] = out->log_surface_pressure;
   const size_t per = (size_t)p->mm * p->n_l;
-  hipLaunchKernelGGL(modal_vordiv_kernel, dim3(blocks(per * (3 * kx + nsp))), dim3(kThreads), 0, s, m);
+  hipLaunchKernelGGL(modal_vordiv_kernel, dim3(blocks(per), 3 * kx + nsp), dim3(kThreads), 0, s, m);
   return hip_result(hipGetLastError());
 }
 }  // namespace
