@@ -15,7 +15,9 @@ from oracle import sht_oracle as so
 pytestmark = pytest.mark.gpu
 
 TOL = 5e-5
-SIGMA = {7: [0.0, 0.14, 0.26, 0.42, 0.6, 0.77, 0.9, 1.0],
+SIGMA = {1: [0.0, 1.0],
+         3: [0.0, 0.3, 0.7, 1.0],
+         7: [0.0, 0.14, 0.26, 0.42, 0.6, 0.77, 0.9, 1.0],
          8: [0.0, 0.05, 0.14, 0.26, 0.42, 0.6, 0.77, 0.9, 1.0],
          16: [0.0, 0.025, 0.05, 0.095, 0.14, 0.2, 0.26, 0.34, 0.42, 0.51, 0.6, 0.685, 0.77, 0.835, 0.9, 0.95, 1.0]}
 
@@ -142,7 +144,8 @@ def test_solid_body_rotation_closed_form(tr):
     assert float(ds.divergence.abs().max()) <= TOL * float(np.abs(vor).max())
 
 
-@pytest.mark.parametrize("trunc,kx,nsp", [(31, 8, 1), (42, 7, 1), (85, 8, 1), (31, 8, 8), (31, 16, 1)])
+@pytest.mark.parametrize("trunc,kx,nsp", [(31, 8, 1), (42, 7, 1), (85, 8, 1), (31, 8, 8), (31, 16, 1), (31, 1, 1),
+                                          (42, 3, 3), (213, 8, 1)])
 def test_dynamics_to_physics_matches_the_oracle(tr, trunc, kx, nsp):
     gg, og = _grids(tr, trunc)
     rng = np.random.default_rng(trunc * 10 + kx)
@@ -164,7 +167,7 @@ def test_dynamics_to_physics_matches_the_oracle(tr, trunc, kx, nsp):
         assert err <= TOL, (name, err)
 
 
-@pytest.mark.parametrize("trunc,kx,nsp", [(31, 8, 1), (85, 8, 1), (42, 7, 7), (42, 16, 1)])
+@pytest.mark.parametrize("trunc,kx,nsp", [(31, 8, 1), (85, 8, 1), (42, 7, 7), (42, 16, 1), (42, 3, 1), (31, 1, 1)])
 def test_physics_to_dynamics_matches_the_oracle(tr, trunc, kx, nsp):
     from jcm_amd.physics_interface import PhysicsState
     gg, og = _grids(tr, trunc)
@@ -185,7 +188,7 @@ def test_physics_to_dynamics_matches_the_oracle(tr, trunc, kx, nsp):
     assert _rel(out.tracers["specific_humidity"], ref.tracers["specific_humidity"]) <= TOL
 
 
-@pytest.mark.parametrize("trunc,kx", [(31, 8), (85, 8), (21, 7), (31, 16)])
+@pytest.mark.parametrize("trunc,kx", [(31, 8), (85, 8), (21, 7), (31, 16), (42, 3), (31, 1)])
 def test_tendency_to_dynamics_matches_the_oracle(tr, trunc, kx):
     import torch
     from jcm_amd.physics_interface import PhysicsTendency
